@@ -1,0 +1,132 @@
+// Device-side scene representation shared by the host runtime (which builds
+// it) and the HIP kernels (which read it).  The reference's flat arrays are
+// uploaded unchanged; next to them the runtime builds a private repack:
+//   * one DPrim record per primitive (type, offsets, both matrices and the
+//     shape's geometry gathered out of point3Fs / vector3Fs), read with
+//     wave-uniform scalar loads in the primitive loop;
+//   * every TriangleMesh BVH re-laid as 32-byte nodes in the reference's own
+//     pre-order (box, skip pointer, leaf triangle range), so the traversal
+//     visits boxes and triangles in exactly the reference's order
+//     (GPURendererKernel.java:1097-1194) with one 32-byte fetch per node
+//     instead of three dependent gathers;
+//   * the triangles of every leaf stored contiguously as (A, B - A, C - A)
+//     in 48 bytes, with the original triangle offset for surface properties.
+#pragma once
+
+#include <stdint.h>
+
+namespace dfk {
+
+constexpr int kPrimFloats = 36;  // DPrim payload size in 32-bit words
+
+// One primitive, 36 words (144 bytes).
+struct DPrim {
+  int32_t type;          // shape type (1 plane, 2 sphere, 3 terrain, 4 triangle, 5 mesh)
+  int32_t shape_offset;  // reference shape offset (planes / spheres / triangles / BVH segment)
+  int32_t surface_offset;
+  int32_t node_root;     // mesh: index of the root in the repacked node array
+  float w2o[12];         // world -> object rows 1..3
+  float o2w[12];         // object -> world rows 1..3
+  float geo[8];          // sphere: pos.xyz, radius; plane: a.xyz, n.xyz; triangle: unused
+};
+
+// Repacked BVH node (pre-order).  box = (min.xyz, max.xyz); next = index of
+// the skip target or -1; tri >= 0: first packed triangle of a leaf;
+// tri < 0: tree node whose left child is node -tri - 2 (a leaf without
+// triangles is stored as a tree node whose left child is its skip target).
+struct DNode {
+  float mn[3];
+  float mx[3];
+  int32_t next;
+  int32_t tri;
+};
+
+// Packed triangle: a.xyz, e0 = b - a, e1 = c - a (computed in float exactly
+// as the reference computes them per test), the reference triangle offset and
+// a last-in-leaf flag.
+struct DTri {
+  float a[3];
+  float e0[3];
+  float e1[3];
+  int32_t tri_offset;
+  int32_t last;
+  int32_t pad;
+};
+
+struct DScene {
+  // unchanged reference arrays
+  const float* point2fs;
+  const float* point3fs;
+  const float* vector3fs;
+  const int32_t* planes;
+  const float* spheres;
+  const float* terrains;
+  const int32_t* triangles;
+  const float* textures;
+  const float* surfaces;
+  const int32_t* primitives;
+  // repack
+  const DPrim* prims;
+  const DNode* nodes;
+  const DTri* tris;
+  int32_t prim_count;
+  int32_t pad0;
+};
+
+struct DCamera {
+  float c[19];
+  float fov_x_tan;  // tan(fovX * 0.5 * PI/180) rounded once to float (host)
+  float fov_y_tan;  // tan(-fovY * 0.5 * PI/180)
+  int32_t pad;
+};
+
+struct DSky {
+  float onb_u[3], onb_v[3], onb_w[3];
+  float sun_dir_world[3];
+  float sun_dir[3];
+  float sun_origin[3];
+  float theta;
+  int32_t sky_active;
+  int32_t sun_active;
+  double zenith_l, zenith_x, zenith_y;
+  double pl[5], px[5], py[5];
+  // Per-frame constants hoisted out of the per-ray sky evaluation
+  // (GPURendererKernel.java:2193-2205), each computed with the same
+  // operation order as the kernel expression it replaces:
+  double l_den;           // whole luminance denominator
+  double x_den0, y_den0;  // (1 + A * exp(B)) of the x / y denominators
+  double x_c, y_c;        // E * cos(theta0) * cos(theta0) of the x / y denominators
+};
+
+struct DParams {
+  int32_t renderer_type;
+  int32_t tone_mapper_type;
+  float exposure;
+  int32_t depth_max;
+  int32_t depth_rr;
+  float ao_max;
+  int32_t normal_mapping;
+  int32_t wireframes;
+  int32_t shader_type;
+  int32_t selected;
+  float ga, gf, gg, gl;
+  int32_t go;
+  int32_t width, height;
+  int32_t passes;
+  int32_t clear_first;
+  // sRGB constants (image4j RGBColorSpace.SRGB)
+  float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
+};
+
+// Per-pixel frame buffers.
+struct DFrame {
+  float* film;             // 3 * W * H
+  int32_t* film_samples;   // W * H
+  float* image_float;      // 3 * W * H
+  uint8_t* image_bgra;     // 4 * W * H
+  int32_t* primitive_offsets;
+  uint64_t* seeds;
+  unsigned long long* counters;  // [rays, samples]
+};
+
+}  // namespace dfk

@@ -1,0 +1,888 @@
+// Host runtime of libdayflower_hip.so: the C ABI of include/dayflower_hip.h.
+//
+// Owns the device copies of the reference's flat arrays, validates every
+// offset they contain before a kernel may dereference it, builds the
+// device-private repack (DPrim / DNode / DTri, df_device_scene.h) and
+// launches the kernels on one HIP stream per handle.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/dayflower_hip.h"
+#include "df_device_scene.h"
+#include "df_kernels.h"
+
+using dfk::DCamera;
+using dfk::DFrame;
+using dfk::DNode;
+using dfk::DParams;
+using dfk::DPrim;
+using dfk::DScene;
+using dfk::DSky;
+using dfk::DTri;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// Host copy of the arrays (kept for re-validation and re-packing after
+// transform / surface updates).
+struct HostScene {
+  std::vector<float> point2fs, point3fs, vector3fs, spheres, terrains, textures, surfaces, o2w, w2o;
+  std::vector<int32_t> planes, triangles, bvh, primitives;
+};
+
+}  // namespace
+
+struct df_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  std::string err;
+
+  HostScene hs;
+  bool has_scene = false;
+  std::vector<DevBuf> scene_bufs;
+  DScene ds{};
+  float camera[19] = {0};
+  df_sky sky{};
+  bool has_sky = false;
+  df_params params{};
+
+  int width = 0, height = 0;
+  DevBuf film, film_samples, image_float, image_bgra, prim_offsets, seeds, counters;
+  bool clear_pending = false;
+};
+
+namespace {
+
+int fail(df_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+int hip_fail(df_ctx* c, hipError_t e, const char* what) {
+  return fail(c, DF_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define DF_HIP(ctx, call)                                   \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return hip_fail((ctx), e_, #call); \
+  } while (0)
+
+void free_buf(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int alloc_buf(df_ctx* c, DevBuf& b, size_t bytes) {
+  free_buf(b);
+  if (bytes == 0) bytes = 4;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) {
+    b.p = nullptr;
+    return fail(c, DF_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  b.bytes = bytes;
+  return DF_OK;
+}
+
+template <typename T>
+int upload(df_ctx* c, DevBuf& b, const std::vector<T>& v) {
+  int rc = alloc_buf(c, b, v.size() * sizeof(T));
+  if (rc) return rc;
+  if (!v.empty()) DF_HIP(c, hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return DF_OK;
+}
+
+// Java (int) cast for offsets stored as floats.
+inline int32_t jf2i(float v) {
+  if (v != v) return 0;
+  if (v >= 2147483648.0f) return INT32_MAX;
+  if (v <= -2147483648.0f) return INT32_MIN;
+  return static_cast<int32_t>(v);
+}
+
+inline bool in_range(int64_t off, int64_t count, size_t len) { return off >= 0 && off + count <= static_cast<int64_t>(len); }
+
+// ------------------------------------------------------------ validation
+
+int check_texture(df_ctx* c, const HostScene& h, int32_t off, bool allow_blend) {
+  if (!in_range(off, 2, h.textures.size())) return fail(c, DF_EINVAL, "texture offset %d out of range", off);
+  const int type = jf2i(h.textures[off]);
+  static const int kSizes[9] = {0, 5, 4, 8, 3, 7, 8, 3, 2};
+  if (type < 1 || type > 8) return DF_OK;  // the kernel returns colour 0 for unknown types
+  int64_t size = kSizes[type];
+  if (type == 6) {
+    if (!in_range(off, 8, h.textures.size())) return fail(c, DF_EINVAL, "image texture at %d truncated", off);
+    const float w = h.textures[off + 4], hh = h.textures[off + 5];
+    if (!(w >= 1.0f && hh >= 1.0f && w <= 65536.0f && hh <= 65536.0f))
+      return fail(c, DF_EINVAL, "image texture at %d has bad size", off);
+    size = 8 + static_cast<int64_t>(jf2i(w * hh));
+  }
+  if (!in_range(off, size, h.textures.size())) return fail(c, DF_EINVAL, "texture at %d truncated", off);
+  if (type == 1) {
+    if (!allow_blend) return DF_OK;  // nested blends are colour 0 in the reference
+    int rc = check_texture(c, h, jf2i(h.textures[off + 2]), false);
+    if (rc) return rc;
+    rc = check_texture(c, h, jf2i(h.textures[off + 3]), false);
+    if (rc) return rc;
+  }
+  return DF_OK;
+}
+
+int check_surface(df_ctx* c, const HostScene& h, int32_t so) {
+  if (!in_range(so, 6, h.surfaces.size())) return fail(c, DF_EINVAL, "surface offset %d out of range", so);
+  for (int k = 1; k <= 3; ++k) {
+    int rc = check_texture(c, h, jf2i(h.surfaces[so + k]), true);
+    if (rc) return rc;
+  }
+  return DF_OK;
+}
+
+int check_triangle(df_ctx* c, const HostScene& h, int32_t to) {
+  if (!in_range(to, 12, h.triangles.size())) return fail(c, DF_EINVAL, "triangle offset %d out of range", to);
+  const int32_t* t = h.triangles.data() + to;
+  for (int k = 0; k < 3; ++k)
+    if (!in_range(t[k], 3, h.point3fs.size())) return fail(c, DF_EINVAL, "triangle %d position offset bad", to);
+  for (int k = 3; k < 6; ++k)
+    if (!in_range(t[k], 3, h.vector3fs.size())) return fail(c, DF_EINVAL, "triangle %d normal offset bad", to);
+  for (int k = 6; k < 9; ++k)
+    if (!in_range(t[k], 2, h.point2fs.size())) return fail(c, DF_EINVAL, "triangle %d uv offset bad", to);
+  for (int k = 9; k < 12; ++k)
+    if (!in_range(t[k], 3, h.vector3fs.size())) return fail(c, DF_EINVAL, "triangle %d tangent offset bad", to);
+  return DF_OK;
+}
+
+DTri pack_triangle(const HostScene& h, int32_t to, bool last) {
+  DTri d{};
+  const int32_t* t = h.triangles.data() + to;
+  const float* A = h.point3fs.data() + t[0];
+  const float* B = h.point3fs.data() + t[1];
+  const float* C = h.point3fs.data() + t[2];
+  for (int k = 0; k < 3; ++k) {
+    d.a[k] = A[k];
+    d.e0[k] = B[k] - A[k];  // the kernel's edge0 (GPURendererKernel.java:1543-1545)
+    d.e1[k] = C[k] - A[k];  // edge1 (:1548-1550)
+  }
+  d.tri_offset = to;
+  d.last = last ? 1 : 0;
+  return d;
+}
+
+// Re-lays one mesh's skip-pointer BVH segment (SceneCompiler.java:892-995)
+// as DNodes in the same pre-order, validating every field.
+int repack_mesh(df_ctx* c, const HostScene& h, int32_t base, std::vector<DNode>& nodes, std::vector<DTri>& tris,
+                int32_t* root_out) {
+  const size_t len = h.bvh.size();
+  std::unordered_map<int32_t, int32_t> index_of;  // relative offset -> node index
+  struct Pending {
+    int32_t node, rel_next, rel_left;
+  };
+  std::vector<Pending> pending;
+  const int32_t first = static_cast<int32_t>(nodes.size());
+  int32_t rel = 0;
+  int64_t open = 1;  // subtrees still to be read (pre-order completeness)
+  while (open > 0) {
+    const int64_t off = static_cast<int64_t>(base) + rel;
+    if (!in_range(off, 5, len)) return fail(c, DF_EINVAL, "BVH node at %lld out of range", static_cast<long long>(off));
+    const int32_t* n = h.bvh.data() + off;
+    const int type = n[0];
+    if (type != 1 && type != 2) return fail(c, DF_EINVAL, "BVH node at %lld has type %d", static_cast<long long>(off), type);
+    if (!in_range(n[2], 3, h.point3fs.size()) || !in_range(n[3], 3, h.point3fs.size()))
+      return fail(c, DF_EINVAL, "BVH node at %lld has bad bound offsets", static_cast<long long>(off));
+    DNode d{};
+    for (int k = 0; k < 3; ++k) {
+      d.mn[k] = h.point3fs[n[2] + k];
+      d.mx[k] = h.point3fs[n[3] + k];
+    }
+    const int32_t idx = static_cast<int32_t>(nodes.size());
+    index_of[rel] = idx;
+    int32_t size = 5;
+    if (type == 1) {
+      d.tri = -1;
+      pending.push_back({idx, n[1], n[4]});
+      open += 1;
+    } else {
+      const int32_t count = n[4];
+      if (count < 0 || !in_range(off, 5 + static_cast<int64_t>(count), len))
+        return fail(c, DF_EINVAL, "BVH leaf at %lld has bad triangle count", static_cast<long long>(off));
+      if (count == 0) {
+        d.tri = -1;
+        pending.push_back({idx, n[1], n[1]});  // descends straight to its skip target
+      } else {
+        d.tri = static_cast<int32_t>(tris.size());
+        for (int32_t k = 0; k < count; ++k) {
+          const int rc = check_triangle(c, h, n[5 + k]);
+          if (rc) return rc;
+          tris.push_back(pack_triangle(h, n[5 + k], k == count - 1));
+        }
+        pending.push_back({idx, n[1], -2});
+      }
+      size = 5 + count;
+      open -= 1;
+    }
+    nodes.push_back(d);
+    rel += size;
+  }
+  for (const Pending& p : pending) {
+    auto resolve = [&](int32_t r, int32_t* out) -> bool {
+      if (r == -1) {
+        *out = -1;
+        return true;
+      }
+      auto it = index_of.find(r);
+      if (it == index_of.end()) return false;
+      *out = it->second;
+      return true;
+    };
+    int32_t next;
+    if (!resolve(p.rel_next, &next)) return fail(c, DF_EINVAL, "BVH skip pointer %d does not name a node", p.rel_next);
+    nodes[p.node].next = next;
+    if (p.rel_left != -2) {
+      int32_t left;
+      if (!resolve(p.rel_left, &left)) return fail(c, DF_EINVAL, "BVH left pointer %d does not name a node", p.rel_left);
+      nodes[p.node].tri = -left - 2;
+    }
+  }
+  // A traversal must terminate: every pointer moves strictly forward in
+  // pre-order except -1, which ends the walk.
+  for (int32_t i = first; i < static_cast<int32_t>(nodes.size()); ++i) {
+    const DNode& d = nodes[i];
+    if (d.next != -1 && d.next <= i) return fail(c, DF_EINVAL, "BVH skip pointer does not advance");
+    if (d.tri < 0) {
+      const int32_t left = -d.tri - 2;
+      if (left != -1 && left <= i) return fail(c, DF_EINVAL, "BVH left pointer does not advance");
+    }
+  }
+  *root_out = first;
+  return DF_OK;
+}
+
+int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::vector<DNode>& nodes,
+                std::vector<DTri>& tris) {
+  const size_t np = h.primitives.size() / 3;
+  if (h.primitives.size() % 3 != 0 || np == 0) return fail(c, DF_EINVAL, "primitives length %zu", h.primitives.size());
+  if (h.o2w.size() < np * 16 || h.w2o.size() < np * 16)
+    return fail(c, DF_EINVAL, "transform arrays hold fewer than 16 floats per primitive");
+  prims.assign(np, DPrim{});
+  for (size_t i = 0; i < np; ++i) {
+    DPrim& p = prims[i];
+    p.type = h.primitives[i * 3 + 0];
+    p.shape_offset = h.primitives[i * 3 + 1];
+    p.surface_offset = h.primitives[i * 3 + 2];
+    p.node_root = -1;
+    int rc = check_surface(c, h, p.surface_offset);
+    if (rc) return rc;
+    for (int k = 0; k < 12; ++k) {
+      p.w2o[k] = h.w2o[i * 16 + k];
+      p.o2w[k] = h.o2w[i * 16 + k];
+    }
+    const int32_t so = p.shape_offset;
+    switch (p.type) {
+      case 1: {  // plane
+        if (!in_range(so, 4, h.planes.size())) return fail(c, DF_EINVAL, "plane offset %d out of range", so);
+        const int32_t* pl = h.planes.data() + so;
+        for (int k = 0; k < 3; ++k)
+          if (!in_range(pl[k], 3, h.point3fs.size())) return fail(c, DF_EINVAL, "plane %d point offset bad", so);
+        if (!in_range(pl[3], 3, h.vector3fs.size())) return fail(c, DF_EINVAL, "plane %d normal offset bad", so);
+        for (int k = 0; k < 3; ++k) {
+          p.geo[k] = h.point3fs[pl[0] + k];
+          p.geo[3 + k] = h.vector3fs[pl[3] + k];
+        }
+        break;
+      }
+      case 2: {  // sphere
+        if (!in_range(so, 2, h.spheres.size())) return fail(c, DF_EINVAL, "sphere offset %d out of range", so);
+        const int32_t po = jf2i(h.spheres[so]);
+        if (!in_range(po, 3, h.point3fs.size())) return fail(c, DF_EINVAL, "sphere %d position offset bad", so);
+        for (int k = 0; k < 3; ++k) p.geo[k] = h.point3fs[po + k];
+        p.geo[3] = h.spheres[so + 1];
+        break;
+      }
+      case 3:  // terrain: its parameters are not read by the march
+        if (!in_range(so, 5, h.terrains.size())) return fail(c, DF_EINVAL, "terrain offset %d out of range", so);
+        break;
+      case 4: {  // triangle
+        rc = check_triangle(c, h, so);
+        if (rc) return rc;
+        p.node_root = static_cast<int32_t>(tris.size());
+        tris.push_back(pack_triangle(h, so, true));
+        break;
+      }
+      case 5: {  // triangle mesh
+        rc = repack_mesh(c, h, so, nodes, tris, &p.node_root);
+        if (rc) return rc;
+        break;
+      }
+      default:
+        break;  // unknown shapes are never hit (the reference skips them)
+    }
+  }
+  return DF_OK;
+}
+
+template <typename T>
+std::vector<T> copy_in(const T* p, size_t n) {
+  return (p && n) ? std::vector<T>(p, p + n) : std::vector<T>();
+}
+
+// Per-frame camera constants: tan(fovX * 0.5 * PI/180) and tan(-fovY * ...)
+// rounded once to float, exactly as run() evaluates them per work-item
+// (GPURendererKernel.java:602-605).
+DCamera make_camera(const float cam[19]) {
+  DCamera d{};
+  std::memcpy(d.c, cam, sizeof d.c);
+  const float pi_180 = static_cast<float>(M_PI) / 180.0f;
+  d.fov_x_tan = static_cast<float>(std::tan(static_cast<double>(cam[4] * 0.5f * pi_180)));
+  d.fov_y_tan = static_cast<float>(std::tan(static_cast<double>(-cam[5] * 0.5f * pi_180)));
+  return d;
+}
+
+DSky make_sky(const df_sky& s) {
+  DSky d{};
+  for (int k = 0; k < 3; ++k) {
+    d.onb_u[k] = s.onb_u[k];
+    d.onb_v[k] = s.onb_v[k];
+    d.onb_w[k] = s.onb_w[k];
+    d.sun_dir_world[k] = s.sun_direction_world[k];
+    d.sun_dir[k] = s.sun_direction[k];
+    d.sun_origin[k] = s.sun_origin[k];
+  }
+  d.theta = s.theta;
+  d.sky_active = s.is_sky_active;
+  d.sun_active = s.is_sun_active;
+  d.zenith_l = s.zenith_relative_luminance;
+  d.zenith_x = s.zenith_x;
+  d.zenith_y = s.zenith_y;
+  for (int k = 0; k < 5; ++k) {
+    d.pl[k] = s.perez_relative_luminance[k];
+    d.px[k] = s.perez_x[k];
+    d.py[k] = s.perez_y[k];
+  }
+  // GPURendererKernel.java:2150-2203, the parts that do not depend on the ray.
+  const double theta0 = static_cast<double>(s.theta);
+  const double ct0 = std::cos(theta0);
+  const double* L = d.pl;
+  d.l_den = ((1.0 + L[0] * std::exp(L[1])) * (1.0 + L[2] * std::exp(L[3] * theta0) + L[4] * ct0 * ct0));
+  d.x_den0 = (1.0 + d.px[0] * std::exp(d.px[1]));
+  d.x_c = d.px[4] * ct0 * ct0;
+  d.y_den0 = (1.0 + d.py[0] * std::exp(d.py[1]));
+  d.y_c = d.py[4] * ct0 * ct0;
+  return d;
+}
+
+inline float jpowf(float a, float b) { return static_cast<float>(std::pow(static_cast<double>(a), static_cast<double>(b))); }
+
+DParams make_params(const df_params& p, int w, int h, int passes, bool clear) {
+  DParams d{};
+  d.renderer_type = p.renderer_type;
+  d.tone_mapper_type = p.tone_mapper_type;
+  d.exposure = p.tone_mapper_exposure;
+  d.depth_max = p.pt_depth_maximum;
+  d.depth_rr = p.pt_depth_russian_roulette;
+  d.ao_max = p.ao_maximum_distance;
+  d.normal_mapping = p.normal_mapping;
+  d.wireframes = p.wireframes;
+  d.shader_type = p.shader_type;
+  d.selected = p.selected_primitive_offset;
+  d.ga = p.global_amplitude;
+  d.gf = p.global_frequency;
+  d.gg = p.global_gain;
+  d.gl = p.global_lacunarity;
+  d.go = p.global_octaves;
+  d.width = w;
+  d.height = h;
+  d.passes = passes;
+  d.clear_first = clear ? 1 : 0;
+  // image4j RGBColorSpace.SRGB (breakPoint 0.00304, gamma 2.4) and its
+  // derived slope / slopeMatch / segmentOffset (RGBColorSpace.java:148-158).
+  const float bp = 0.00304f, gamma = 2.4f;
+  const float slope = 1.0f / (gamma / jpowf(bp, 1.0f / gamma - 1.0f) - gamma * bp + bp);
+  const float slope_match = gamma * slope / jpowf(bp, 1.0f / gamma - 1.0f);
+  d.srgb_break = bp;
+  d.srgb_gamma_inv = 1.0f / gamma;
+  d.srgb_slope = slope;
+  d.srgb_slope_match = slope_match;
+  d.srgb_segment = slope_match * jpowf(bp, 1.0f / gamma) - slope * bp;
+  return d;
+}
+
+int validate_params(df_ctx* c, const df_params& p) {
+  if (p.renderer_type < 1 || p.renderer_type > 6) return fail(c, DF_EINVAL, "renderer_type %d", p.renderer_type);
+  if (p.tone_mapper_type < 1 || p.tone_mapper_type > 4) return fail(c, DF_EINVAL, "tone_mapper_type %d", p.tone_mapper_type);
+  if (p.shader_type < 1 || p.shader_type > 2) return fail(c, DF_EINVAL, "shader_type %d", p.shader_type);
+  // Any depth is accepted as in the reference (a value below 1 still traces
+  // one bounce); the upper bound keeps a path with unit albedo everywhere
+  // from looping for billions of bounces.
+  if (p.pt_depth_maximum > 256) return fail(c, DF_EINVAL, "pt_depth_maximum %d > 256", p.pt_depth_maximum);
+  if (p.global_octaves < 0 || p.global_octaves > 64) return fail(c, DF_EINVAL, "global_octaves %d", p.global_octaves);
+  if (p.math_mode != DF_MATH_FAST && p.math_mode != DF_MATH_EXACT) return fail(c, DF_EINVAL, "math_mode %d", p.math_mode);
+  return DF_OK;
+}
+
+df_params reference_defaults() {
+  df_params p{};
+  p.renderer_type = DF_RENDERER_PATH_TRACER;
+  p.tone_mapper_type = DF_TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED;
+  p.tone_mapper_exposure = 1.0f;
+  p.pt_depth_maximum = 5;
+  p.pt_depth_russian_roulette = 5;
+  p.ao_maximum_distance = 200.0f;
+  p.normal_mapping = 1;
+  p.wireframes = 0;
+  p.shader_type = DF_SHADER_GOURAUD;
+  p.selected_primitive_offset = -1;
+  p.global_amplitude = 0.5f;
+  p.global_frequency = 0.2f;
+  p.global_gain = 0.5f;
+  p.global_lacunarity = 2.0f;
+  p.global_octaves = 2;
+  p.math_mode = DF_MATH_FAST;
+  return p;
+}
+
+int rebuild_device_scene(df_ctx* c) {
+  std::vector<DPrim> prims;
+  std::vector<DNode> nodes;
+  std::vector<DTri> tris;
+  int rc = build_prims(c, c->hs, prims, nodes, tris);
+  if (rc) return rc;
+  // buffers: 0 point2fs 1 point3fs 2 vector3fs 3 planes 4 spheres 5 terrains
+  // 6 triangles 7 textures 8 surfaces 9 primitives 10 prims 11 nodes 12 tris
+  c->scene_bufs.resize(13);
+  const HostScene& h = c->hs;
+  if ((rc = upload(c, c->scene_bufs[0], h.point2fs)) || (rc = upload(c, c->scene_bufs[1], h.point3fs)) ||
+      (rc = upload(c, c->scene_bufs[2], h.vector3fs)) || (rc = upload(c, c->scene_bufs[3], h.planes)) ||
+      (rc = upload(c, c->scene_bufs[4], h.spheres)) || (rc = upload(c, c->scene_bufs[5], h.terrains)) ||
+      (rc = upload(c, c->scene_bufs[6], h.triangles)) || (rc = upload(c, c->scene_bufs[7], h.textures)) ||
+      (rc = upload(c, c->scene_bufs[8], h.surfaces)) || (rc = upload(c, c->scene_bufs[9], h.primitives)) ||
+      (rc = upload(c, c->scene_bufs[10], prims)) || (rc = upload(c, c->scene_bufs[11], nodes)) ||
+      (rc = upload(c, c->scene_bufs[12], tris)))
+    return rc;
+  DScene& d = c->ds;
+  d.point2fs = static_cast<const float*>(c->scene_bufs[0].p);
+  d.point3fs = static_cast<const float*>(c->scene_bufs[1].p);
+  d.vector3fs = static_cast<const float*>(c->scene_bufs[2].p);
+  d.planes = static_cast<const int32_t*>(c->scene_bufs[3].p);
+  d.spheres = static_cast<const float*>(c->scene_bufs[4].p);
+  d.terrains = static_cast<const float*>(c->scene_bufs[5].p);
+  d.triangles = static_cast<const int32_t*>(c->scene_bufs[6].p);
+  d.textures = static_cast<const float*>(c->scene_bufs[7].p);
+  d.surfaces = static_cast<const float*>(c->scene_bufs[8].p);
+  d.primitives = static_cast<const int32_t*>(c->scene_bufs[9].p);
+  d.prims = static_cast<const DPrim*>(c->scene_bufs[10].p);
+  d.nodes = static_cast<const DNode*>(c->scene_bufs[11].p);
+  d.tris = static_cast<const DTri*>(c->scene_bufs[12].p);
+  d.prim_count = static_cast<int32_t>(prims.size());
+  return DF_OK;
+}
+
+int need_frame(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  if (c->width <= 0 || !c->film.p) return fail(c, DF_ESTATE, "df_update has not been called");
+  return DF_OK;
+}
+
+size_t pixels(const df_ctx* c) { return static_cast<size_t>(c->width) * static_cast<size_t>(c->height); }
+
+int read_back(df_ctx* c, void* dst, const DevBuf& b, size_t bytes) {
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  DF_HIP(c, hipMemcpy(dst, b.p, bytes, hipMemcpyDeviceToHost));
+  return DF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int df_device_count(int32_t* count) {
+  if (!count) return DF_EINVAL;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return DF_OK;
+}
+
+int df_validate_scene(const df_compiled_scene* s, char* msg, size_t cap) {
+  if (!s) return DF_EINVAL;
+  df_ctx scratch;  // error holder only; no HIP call is made
+  int rc = DF_OK;
+  if (!s->camera || s->camera_len != 19) {
+    rc = fail(&scratch, DF_EINVAL, "camera must hold 19 floats");
+  } else if (!s->primitives || s->primitives_len < 3) {
+    rc = fail(&scratch, DF_EINVAL, "scene has no primitives");
+  } else {
+    HostScene h;
+    h.point2fs = copy_in(s->point2fs, s->point2fs_len);
+    h.point3fs = copy_in(s->point3fs, s->point3fs_len);
+    h.vector3fs = copy_in(s->vector3fs, s->vector3fs_len);
+    h.planes = copy_in(s->planes, s->planes_len);
+    h.spheres = copy_in(s->spheres, s->spheres_len);
+    h.terrains = copy_in(s->terrains, s->terrains_len);
+    h.triangles = copy_in(s->triangles, s->triangles_len);
+    h.bvh = copy_in(s->bvh, s->bvh_len);
+    h.textures = copy_in(s->textures, s->textures_len);
+    h.surfaces = copy_in(s->surfaces, s->surfaces_len);
+    h.primitives = copy_in(s->primitives, s->primitives_len);
+    h.o2w = copy_in(s->object_to_world, s->object_to_world_len);
+    h.w2o = copy_in(s->world_to_object, s->world_to_object_len);
+    std::vector<DPrim> prims;
+    std::vector<DNode> nodes;
+    std::vector<DTri> tris;
+    rc = build_prims(&scratch, h, prims, nodes, tris);
+  }
+  if (msg && cap) {
+    std::snprintf(msg, cap, "%s", rc ? scratch.err.c_str() : "ok");
+  }
+  return rc;
+}
+
+int df_create(df_ctx** out, int32_t device) {
+  if (!out) return DF_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DF_ENODEVICE;
+  if (device < 0 || device >= n) return DF_ENODEVICE;
+  df_ctx* c = new (std::nothrow) df_ctx;
+  if (!c) return DF_ENOMEM;
+  c->device = device;
+  c->params = reference_defaults();
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    df_destroy(c);
+    return DF_EHIP;
+  }
+  if (alloc_buf(c, c->counters, 2 * sizeof(unsigned long long)) != DF_OK ||
+      hipMemset(c->counters.p, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    df_destroy(c);
+    return DF_ENOMEM;
+  }
+  *out = c;
+  return DF_OK;
+}
+
+void df_destroy(df_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (DevBuf& b : c->scene_bufs) free_buf(b);
+  free_buf(c->film);
+  free_buf(c->film_samples);
+  free_buf(c->image_float);
+  free_buf(c->image_bgra);
+  free_buf(c->prim_offsets);
+  free_buf(c->seeds);
+  free_buf(c->counters);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int df_load_scene(df_ctx* c, const df_compiled_scene* s) {
+  if (!c || !s) return DF_EINVAL;
+  if (!s->camera || s->camera_len != 19) return fail(c, DF_EINVAL, "camera must hold 19 floats");
+  if (!s->primitives || s->primitives_len < 3) return fail(c, DF_EINVAL, "scene has no primitives");
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  HostScene h;
+  h.point2fs = copy_in(s->point2fs, s->point2fs_len);
+  h.point3fs = copy_in(s->point3fs, s->point3fs_len);
+  h.vector3fs = copy_in(s->vector3fs, s->vector3fs_len);
+  h.planes = copy_in(s->planes, s->planes_len);
+  h.spheres = copy_in(s->spheres, s->spheres_len);
+  h.terrains = copy_in(s->terrains, s->terrains_len);
+  h.triangles = copy_in(s->triangles, s->triangles_len);
+  h.bvh = copy_in(s->bvh, s->bvh_len);
+  h.textures = copy_in(s->textures, s->textures_len);
+  h.surfaces = copy_in(s->surfaces, s->surfaces_len);
+  h.primitives = copy_in(s->primitives, s->primitives_len);
+  h.o2w = copy_in(s->object_to_world, s->object_to_world_len);
+  h.w2o = copy_in(s->world_to_object, s->world_to_object_len);
+  std::swap(c->hs, h);
+  const int rc = rebuild_device_scene(c);
+  if (rc) {
+    std::swap(c->hs, h);
+    c->has_scene = false;
+    return rc;
+  }
+  std::memcpy(c->camera, s->camera, sizeof c->camera);
+  c->has_scene = true;
+  return DF_OK;
+}
+
+int df_update(df_ctx* c, int32_t w, int32_t h, int32_t local_size_hint) {
+  (void)local_size_hint;
+  if (!c) return DF_EINVAL;
+  if (w <= 0 || h <= 0 || static_cast<int64_t>(w) * h > (1ll << 28)) return fail(c, DF_EINVAL, "bad resolution %dx%d", w, h);
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t n = static_cast<size_t>(w) * h;
+  int rc;
+  if ((rc = alloc_buf(c, c->film, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->film_samples, n * sizeof(int32_t))) ||
+      (rc = alloc_buf(c, c->image_float, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->image_bgra, n * 4)) ||
+      (rc = alloc_buf(c, c->prim_offsets, n * sizeof(int32_t))) || (rc = alloc_buf(c, c->seeds, n * sizeof(uint64_t))))
+    return rc;
+  DF_HIP(c, hipMemset(c->film.p, 0, n * 3 * sizeof(float)));
+  DF_HIP(c, hipMemset(c->film_samples.p, 0, n * sizeof(int32_t)));
+  DF_HIP(c, hipMemset(c->image_float.p, 0, n * 3 * sizeof(float)));
+  DF_HIP(c, hipMemset(c->image_bgra.p, 0, n * 4));
+  DF_HIP(c, hipMemset(c->prim_offsets.p, 0xFF, n * sizeof(int32_t)));  // Arrays.fill(primitiveOffsets, -1)
+  c->width = w;
+  c->height = h;
+  c->clear_pending = false;
+  DF_HIP(c, dfk_launch_seed(static_cast<uint64_t*>(c->seeds.p), n, 0, c->stream));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  return DF_OK;
+}
+
+int df_seed(df_ctx* c, uint64_t master) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, dfk_launch_seed(static_cast<uint64_t*>(c->seeds.p), pixels(c), master, c->stream));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  return DF_OK;
+}
+
+int df_set_seeds(df_ctx* c, const int64_t* seeds, size_t n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!seeds || n != pixels(c)) return fail(c, DF_EINVAL, "seed count %zu != %zu", n, pixels(c));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  DF_HIP(c, hipMemcpy(c->seeds.p, seeds, n * sizeof(int64_t), hipMemcpyHostToDevice));
+  return DF_OK;
+}
+
+int df_get_seeds(df_ctx* c, int64_t* seeds, size_t n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!seeds || n != pixels(c)) return fail(c, DF_EINVAL, "seed count %zu != %zu", n, pixels(c));
+  return read_back(c, seeds, c->seeds, n * sizeof(int64_t));
+}
+
+int df_upload_camera(df_ctx* c, const float* cam, size_t n) {
+  if (!c || !cam) return DF_EINVAL;
+  if (n != 19) return fail(c, DF_EINVAL, "camera must hold 19 floats");
+  std::memcpy(c->camera, cam, sizeof c->camera);
+  return DF_OK;
+}
+
+int df_upload_transforms(df_ctx* c, const float* o2w, const float* w2o, int32_t n_prim) {
+  if (!c || !o2w || !w2o) return DF_EINVAL;
+  if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
+  if (n_prim < 0 || static_cast<size_t>(n_prim) * 3 != c->hs.primitives.size())
+    return fail(c, DF_EINVAL, "transform count %d != primitive count %zu", n_prim, c->hs.primitives.size() / 3);
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  c->hs.o2w.assign(o2w, o2w + static_cast<size_t>(n_prim) * 16);
+  c->hs.w2o.assign(w2o, w2o + static_cast<size_t>(n_prim) * 16);
+  return rebuild_device_scene(c);
+}
+
+int df_upload_surfaces(df_ctx* c, const float* surfaces, size_t n) {
+  if (!c || !surfaces) return DF_EINVAL;
+  if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<float> old = c->hs.surfaces;
+  c->hs.surfaces.assign(surfaces, surfaces + n);
+  const int rc = rebuild_device_scene(c);
+  if (rc) {
+    c->hs.surfaces.swap(old);
+    (void)rebuild_device_scene(c);
+  }
+  return rc;
+}
+
+int df_upload_sky(df_ctx* c, const df_sky* sky) {
+  if (!c || !sky) return DF_EINVAL;
+  c->sky = *sky;
+  c->has_sky = true;
+  return DF_OK;
+}
+
+int df_toggle_sky(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  c->sky.is_sky_active = c->sky.is_sky_active == 0 ? 1 : 0;
+  return DF_OK;
+}
+
+int df_toggle_sun(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  c->sky.is_sun_active = c->sky.is_sun_active == 0 ? 1 : 0;
+  return DF_OK;
+}
+
+int df_set_params(df_ctx* c, const df_params* p) {
+  if (!c || !p) return DF_EINVAL;
+  const int rc = validate_params(c, *p);
+  if (rc) return rc;
+  c->params = *p;
+  return DF_OK;
+}
+
+int df_get_params(df_ctx* c, df_params* p) {
+  if (!c || !p) return DF_EINVAL;
+  *p = c->params;
+  return DF_OK;
+}
+
+int df_toggle_primitive_selection(df_ctx* c, int32_t x, int32_t y) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  const int64_t index = static_cast<int64_t>(y) * c->width + x;
+  if (index < 0 || index >= static_cast<int64_t>(pixels(c))) return DF_OK;  // outside: no change (:358)
+  int32_t po = -1;
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  DF_HIP(c, hipMemcpy(&po, static_cast<int32_t*>(c->prim_offsets.p) + index, sizeof po, hipMemcpyDeviceToHost));
+  c->params.selected_primitive_offset = po == c->params.selected_primitive_offset ? -1 : po;
+  return DF_OK;
+}
+
+int df_toggle_primitive_material(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  const int32_t sel = c->params.selected_primitive_offset;
+  if (sel == -1 || !c->has_scene) return DF_OK;
+  if (!in_range(sel, 3, c->hs.primitives.size())) return fail(c, DF_EINVAL, "selected primitive %d out of range", sel);
+  const int32_t so = c->hs.primitives[sel + 2];
+  const int old_type = jf2i(c->hs.surfaces[so]);
+  const int ord = old_type - 1;
+  if (ord < 0 || ord >= 5) return DF_OK;
+  std::vector<float> s = c->hs.surfaces;
+  s[so] = static_cast<float>((ord + 1) % 5 + 1);  // ClearCoat, Glass, Lambertian, Phong, Reflection
+  return df_upload_surfaces(c, s.data(), s.size());
+}
+
+int df_clear(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  c->clear_pending = true;
+  return DF_OK;
+}
+
+int df_execute(df_ctx* c, int32_t passes) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
+  if (!c->has_sky) return fail(c, DF_ESTATE, "no sky uploaded");
+  if (passes < 0) return fail(c, DF_EINVAL, "passes %d", passes);
+  if (passes == 0) return DF_OK;
+  DF_HIP(c, hipSetDevice(c->device));
+  const DCamera cam = make_camera(c->camera);
+  const DSky sky = make_sky(c->sky);
+  const DParams par = make_params(c->params, c->width, c->height, passes, c->clear_pending);
+  DFrame f{};
+  f.film = static_cast<float*>(c->film.p);
+  f.film_samples = static_cast<int32_t*>(c->film_samples.p);
+  f.image_float = static_cast<float*>(c->image_float.p);
+  f.image_bgra = static_cast<uint8_t*>(c->image_bgra.p);
+  f.primitive_offsets = static_cast<int32_t*>(c->prim_offsets.p);
+  f.seeds = static_cast<uint64_t*>(c->seeds.p);
+  f.counters = static_cast<unsigned long long*>(c->counters.p);
+  DF_HIP(c, hipEventRecord(c->ev0, c->stream));
+  DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, c->params.math_mode, c->stream));
+  DF_HIP(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  c->clear_pending = false;  // clearFilmFlags() after the pass
+  return DF_OK;
+}
+
+int df_synchronize(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  return DF_OK;
+}
+
+int df_last_execute_ms(df_ctx* c, float* ms) {
+  if (!c || !ms) return DF_EINVAL;
+  if (!c->timed) return fail(c, DF_ESTATE, "nothing executed yet");
+  DF_HIP(c, hipEventSynchronize(c->ev1));
+  DF_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return DF_OK;
+}
+
+int df_get_image_bgra(df_ctx* c, uint8_t* out, size_t n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!out || n != pixels(c) * 4)
+    return fail(c, DF_EINVAL, "resolutionX * resolutionY * 4 != length: %d, %d, %zu", c->width, c->height, n);
+  return read_back(c, out, c->image_bgra, n);
+}
+
+int df_get_image_float(df_ctx* c, float* out, size_t n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!out || n != pixels(c) * 3) return fail(c, DF_EINVAL, "image float length %zu != %zu", n, pixels(c) * 3);
+  return read_back(c, out, c->image_float, n * sizeof(float));
+}
+
+int df_get_primitive_offsets(df_ctx* c, int32_t* out, size_t n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!out || n != pixels(c)) return fail(c, DF_EINVAL, "primitive offsets length %zu != %zu", n, pixels(c));
+  return read_back(c, out, c->prim_offsets, n * sizeof(int32_t));
+}
+
+int df_get_film(df_ctx* c, float* rgb, size_t rgb_n, int32_t* samples, size_t samples_n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!rgb || rgb_n != pixels(c) * 3 || !samples || samples_n != pixels(c))
+    return fail(c, DF_EINVAL, "film lengths %zu / %zu do not match %zu pixels", rgb_n, samples_n, pixels(c));
+  rc = read_back(c, rgb, c->film, rgb_n * sizeof(float));
+  if (rc) return rc;
+  return read_back(c, samples, c->film_samples, samples_n * sizeof(int32_t));
+}
+
+int df_set_film(df_ctx* c, const float* rgb, size_t rgb_n, const int32_t* samples, size_t samples_n) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!rgb || rgb_n != pixels(c) * 3 || !samples || samples_n != pixels(c))
+    return fail(c, DF_EINVAL, "film lengths %zu / %zu do not match %zu pixels", rgb_n, samples_n, pixels(c));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  DF_HIP(c, hipMemcpy(c->film.p, rgb, rgb_n * sizeof(float), hipMemcpyHostToDevice));
+  DF_HIP(c, hipMemcpy(c->film_samples.p, samples, samples_n * sizeof(int32_t), hipMemcpyHostToDevice));
+  return DF_OK;
+}
+
+int df_get_counters(df_ctx* c, df_counters* out) {
+  if (!c || !out) return DF_EINVAL;
+  unsigned long long v[2] = {0, 0};
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  DF_HIP(c, hipMemcpy(v, c->counters.p, sizeof v, hipMemcpyDeviceToHost));
+  out->rays = v[0];
+  out->samples = v[1];
+  return DF_OK;
+}
+
+int df_reset_counters(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  DF_HIP(c, hipMemset(c->counters.p, 0, 2 * sizeof(unsigned long long)));
+  return DF_OK;
+}
+
+const char* df_last_error(df_ctx* c) {
+  if (!c) return "null handle";
+  return c->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,24 @@
+"""MI355X renderer for Dayflower's RendererKernel hot path.
+
+``scene``    - compiled scenes (flat CompiledScene arrays) and their builders.
+``renderer`` - HIPRendererKernel, the renderer-kernel interface over the C ABI.
+"""
+
+from . import _ffi  # noqa: F401
+from .renderer import (  # noqa: F401
+    HIPError,
+    HIPRendererKernel,
+    MATH_EXACT,
+    MATH_FAST,
+    device_count,
+)
+from .scene import (  # noqa: F401
+    CompiledScene,
+    build_batch_scene,
+    build_config,
+    camera_setup,
+    default_params,
+    pixel_seeds,
+    read_scene,
+    sky_set,
+)

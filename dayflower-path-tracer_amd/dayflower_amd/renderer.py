@@ -1,0 +1,341 @@
+"""Python mirror of the reference renderer-kernel interface over the C ABI.
+
+``HIPRendererKernel`` has the method names, argument meaning and error
+behaviour of the reference's AbstractRendererKernel
+(kernel/AbstractRendererKernel.java:36-727) and GPURendererKernel
+(kernel/GPURendererKernel.java:166-569), plus the Aparapi Kernel calls the
+application makes (execute(range), get(...), dispose()).  Every call goes to
+libdayflower_hip.so; there is no CPU path.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _ffi
+from .scene import CompiledScene
+
+BOOLEAN_FALSE = 0
+BOOLEAN_TRUE = 1
+
+RENDERER_TYPE_AMBIENT_OCCLUSION = 1
+RENDERER_TYPE_PATH_TRACER = 2
+RENDERER_TYPE_RAY_CASTER = 3
+RENDERER_TYPE_RAY_MARCHER = 4
+RENDERER_TYPE_RAY_TRACER = 5
+RENDERER_TYPE_SURFACE_NORMALS = 6
+
+SHADER_TYPE_FLAT = 1
+SHADER_TYPE_GOURAUD = 2
+
+TONE_MAPPER_TYPE_FILMIC_CURVE_ACES_MODIFIED = 1
+TONE_MAPPER_TYPE_REINHARD = 2
+TONE_MAPPER_TYPE_REINHARD_MODIFIED_1 = 3
+TONE_MAPPER_TYPE_REINHARD_MODIFIED_2 = 4
+
+MATH_FAST = 0
+MATH_EXACT = 1
+
+
+class HIPError(RuntimeError):
+    """A failed C-ABI call; ``code`` is the DF_E* status."""
+
+    def __init__(self, code, message):
+        super().__init__("%s: %s" % (_ffi.ERROR_NAMES.get(code, code), message))
+        self.code = code
+
+
+def device_count():
+    n = ctypes.c_int32()
+    _ffi.hip_lib().df_device_count(ctypes.byref(n))
+    return n.value
+
+
+class HIPRendererKernel:
+    """Renderer bound to one compiled scene on one HIP device."""
+
+    def __init__(self, scene, device=0):
+        if not isinstance(scene, CompiledScene):
+            raise TypeError("scene must be a CompiledScene")
+        self._lib = _ffi.hip_lib()
+        self._ctx = ctypes.c_void_p()
+        rc = self._lib.df_create(ctypes.byref(self._ctx), int(device))
+        if rc != _ffi.DF_OK:
+            self._ctx = None
+            raise HIPError(rc, "df_create(device=%d)" % device)
+        self._scene = scene
+        self._image = None
+        self.width = 0
+        self.height = 0
+        self._changed = False
+        self._check(self._lib.df_load_scene(self._ctx, ctypes.byref(scene.as_struct())), "df_load_scene")
+        self._check(self._lib.df_upload_sky(self._ctx, ctypes.byref(scene.sky)), "df_upload_sky")
+        self._params = _ffi.Params()
+        self._check(self._lib.df_get_params(self._ctx, ctypes.byref(self._params)), "df_get_params")
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc, what):
+        if rc != _ffi.DF_OK:
+            msg = self._lib.df_last_error(self._ctx) if self._ctx else b""
+            raise HIPError(rc, "%s: %s" % (what, (msg or b"").decode()))
+
+    def _push_params(self):
+        self._check(self._lib.df_set_params(self._ctx, ctypes.byref(self._params)), "df_set_params")
+        self._changed = True
+
+    def dispose(self):
+        """Kernel.dispose(): releases the device resources."""
+        if self._ctx:
+            self._lib.df_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.dispose()
+
+    # ------------------------------------------------------------ lifecycle
+    def update(self, resolutionX, resolutionY, imageDataByte=None, localSize=256):
+        """update(W, H, byte[], localSize) (GPURendererKernel.java:411-449).
+
+        Raises ValueError when ``W * H * 4 != len(imageDataByte)``
+        (AbstractImageKernel.java:146-148)."""
+        n = int(resolutionX) * int(resolutionY)
+        if imageDataByte is None:
+            imageDataByte = np.zeros(n * 4, np.uint8)
+        if n * 4 != len(imageDataByte):
+            raise ValueError("resolutionX * resolutionY * 4 != imageDataByte.length: resolutionX=%d, resolutionY=%d, "
+                             "imageDataByte.length=%d" % (resolutionX, resolutionY, len(imageDataByte)))
+        self._check(self._lib.df_update(self._ctx, int(resolutionX), int(resolutionY), int(localSize)), "df_update")
+        self.width, self.height = int(resolutionX), int(resolutionY)
+        self._image = imageDataByte
+
+    def seed(self, master):
+        """Deterministic per-pixel seeds (replaces ThreadLocalRandom)."""
+        self._check(self._lib.df_seed(self._ctx, ctypes.c_uint64(master)), "df_seed")
+
+    def setSeeds(self, seeds):
+        s = np.ascontiguousarray(seeds, np.int64)
+        self._check(self._lib.df_set_seeds(self._ctx, s.ctypes.data_as(_ffi.c_int64_p), s.size), "df_set_seeds")
+
+    def getSeeds(self):
+        s = np.zeros(self.width * self.height, np.int64)
+        self._check(self._lib.df_get_seeds(self._ctx, s.ctypes.data_as(_ffi.c_int64_p), s.size), "df_get_seeds")
+        return s
+
+    def updateCamera(self, camera=None):
+        """updateCamera() (GPURendererKernel.java:455-461)."""
+        cam = np.ascontiguousarray(self._scene.camera if camera is None else camera, np.float32)
+        self._check(self._lib.df_upload_camera(self._ctx, cam.ctypes.data_as(_ffi.c_float_p), cam.size), "df_upload_camera")
+
+    def updatePrimitives(self, objectToWorld, worldToObject):
+        """updatePrimitives() (GPURendererKernel.java:467-521)."""
+        o = np.ascontiguousarray(objectToWorld, np.float32)
+        w = np.ascontiguousarray(worldToObject, np.float32)
+        self._check(self._lib.df_upload_transforms(self._ctx, o.ctypes.data_as(_ffi.c_float_p),
+                                                   w.ctypes.data_as(_ffi.c_float_p), o.size // 16),
+                    "df_upload_transforms")
+        self._changed = True
+
+    def updateSurfaces(self, surfaces):
+        s = np.ascontiguousarray(surfaces, np.float32)
+        self._check(self._lib.df_upload_surfaces(self._ctx, s.ctypes.data_as(_ffi.c_float_p), s.size),
+                    "df_upload_surfaces")
+        self._changed = True
+
+    def updateSunAndSky(self, sky=None):
+        """updateSunAndSky() (GPURendererKernel.java:527-569)."""
+        self._check(self._lib.df_upload_sky(self._ctx, ctypes.byref(self._scene.sky if sky is None else sky)),
+                    "df_upload_sky")
+        self._changed = True
+
+    # ------------------------------------------------------------ passes
+    def clear(self):
+        """clear(): the next pass overwrites the film (AbstractImageKernel.java:119-121)."""
+        self._check(self._lib.df_clear(self._ctx), "df_clear")
+
+    def clearFilmFlags(self):
+        """clearFilmFlags(): folded into execute (each execute ends a pass)."""
+
+    def execute(self, passes=1):
+        """execute(range) + clearFilmFlags(), ``passes`` times (asynchronous)."""
+        self._check(self._lib.df_execute(self._ctx, int(passes)), "df_execute")
+
+    def synchronize(self):
+        self._check(self._lib.df_synchronize(self._ctx), "df_synchronize")
+
+    def lastExecuteMillis(self):
+        ms = ctypes.c_float()
+        self._check(self._lib.df_last_execute_ms(self._ctx, ctypes.byref(ms)), "df_last_execute_ms")
+        return ms.value
+
+    # ------------------------------------------------------------ readback
+    def get(self):
+        """get(imageDataByte): copies the BGRA image into the array given to update()."""
+        buf = np.zeros(self.width * self.height * 4, np.uint8)
+        self._check(self._lib.df_get_image_bgra(self._ctx, buf.ctypes.data_as(_ffi.c_uint8_p), buf.size),
+                    "df_get_image_bgra")
+        if isinstance(self._image, np.ndarray):
+            self._image[:] = buf
+        else:
+            self._image[:] = buf.tobytes()
+        return self._image
+
+    def getImageBGRA(self):
+        buf = np.zeros(self.width * self.height * 4, np.uint8)
+        self._check(self._lib.df_get_image_bgra(self._ctx, buf.ctypes.data_as(_ffi.c_uint8_p), buf.size),
+                    "df_get_image_bgra")
+        return buf
+
+    def getImageFloat(self):
+        buf = np.zeros(self.width * self.height * 3, np.float32)
+        self._check(self._lib.df_get_image_float(self._ctx, buf.ctypes.data_as(_ffi.c_float_p), buf.size),
+                    "df_get_image_float")
+        return buf
+
+    def getPrimitiveOffsets(self):
+        """get(primitiveOffsets): the ray-cast buffer."""
+        buf = np.zeros(self.width * self.height, np.int32)
+        self._check(self._lib.df_get_primitive_offsets(self._ctx, buf.ctypes.data_as(_ffi.c_int32_p), buf.size),
+                    "df_get_primitive_offsets")
+        return buf
+
+    def getFilm(self):
+        rgb = np.zeros(self.width * self.height * 3, np.float32)
+        n = np.zeros(self.width * self.height, np.int32)
+        self._check(self._lib.df_get_film(self._ctx, rgb.ctypes.data_as(_ffi.c_float_p), rgb.size,
+                                          n.ctypes.data_as(_ffi.c_int32_p), n.size), "df_get_film")
+        return rgb, n
+
+    def setFilm(self, rgb, samples):
+        rgb = np.ascontiguousarray(rgb, np.float32)
+        n = np.ascontiguousarray(samples, np.int32)
+        self._check(self._lib.df_set_film(self._ctx, rgb.ctypes.data_as(_ffi.c_float_p), rgb.size,
+                                          n.ctypes.data_as(_ffi.c_int32_p), n.size), "df_set_film")
+
+    def counters(self):
+        c = _ffi.Counters()
+        self._check(self._lib.df_get_counters(self._ctx, ctypes.byref(c)), "df_get_counters")
+        return {"rays": c.rays, "samples": c.samples}
+
+    def resetCounters(self):
+        self._check(self._lib.df_reset_counters(self._ctx), "df_reset_counters")
+
+    # ------------------------------------------------------------ setters (AbstractRendererKernel.java:392-575)
+    # Invalid enum values are ignored, as in the reference.
+    def setRendererType(self, rendererType):
+        if rendererType in (1, 2, 3, 4, 5, 6) and rendererType != self._params.renderer_type:
+            self._params.renderer_type = rendererType
+            self._push_params()
+
+    def getRendererType(self):
+        return self._params.renderer_type
+
+    def setToneMapperType(self, toneMapperType):
+        if toneMapperType in (1, 2, 3, 4) and toneMapperType != self._params.tone_mapper_type:
+            self._params.tone_mapper_type = toneMapperType
+            self._push_params()
+
+    def getToneMapperType(self):
+        return self._params.tone_mapper_type
+
+    def setToneMapperExposure(self, exposure):
+        self._params.tone_mapper_exposure = float(exposure)
+        self._push_params()
+
+    def setRendererPTRayDepthMaximum(self, depth):
+        self._params.pt_depth_maximum = int(depth)
+        self._push_params()
+
+    def getRendererPTRayDepthMaximum(self):
+        return self._params.pt_depth_maximum
+
+    def setRendererPTRayDepthRussianRoulette(self, depth):
+        self._params.pt_depth_russian_roulette = int(depth)
+        self._push_params()
+
+    def setRendererAOMaximumDistance(self, distance):
+        self._params.ao_maximum_distance = float(distance)
+        self._push_params()
+
+    def setRendererNormalMapping(self, value):
+        if value in (BOOLEAN_FALSE, BOOLEAN_TRUE):
+            self._params.normal_mapping = value
+            self._push_params()
+
+    def setRendererWireframes(self, value):
+        if value in (BOOLEAN_FALSE, BOOLEAN_TRUE):
+            self._params.wireframes = value
+            self._push_params()
+
+    def setShaderType(self, shaderType):
+        if shaderType in (SHADER_TYPE_FLAT, SHADER_TYPE_GOURAUD):
+            self._params.shader_type = shaderType
+            self._push_params()
+
+    def setGlobalAmplitude(self, v):
+        self._params.global_amplitude = float(v)
+        self._push_params()
+
+    def setGlobalFrequency(self, v):
+        self._params.global_frequency = float(v)
+        self._push_params()
+
+    def setGlobalGain(self, v):
+        self._params.global_gain = float(v)
+        self._push_params()
+
+    def setGlobalLacunarity(self, v):
+        self._params.global_lacunarity = float(v)
+        self._push_params()
+
+    def setGlobalOctaves(self, v):
+        self._params.global_octaves = int(v)
+        self._push_params()
+
+    def setMathMode(self, mode):
+        """MATH_FAST (device float library) or MATH_EXACT (double-rounded transcendentals)."""
+        self._params.math_mode = int(mode)
+        self._push_params()
+
+    def setParams(self, params):
+        ctypes.memmove(ctypes.byref(self._params), ctypes.byref(params), ctypes.sizeof(_ffi.Params))
+        self._push_params()
+
+    def getParams(self):
+        p = _ffi.Params()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(self._params), ctypes.sizeof(_ffi.Params))
+        return p
+
+    def toggleSky(self):
+        self._check(self._lib.df_toggle_sky(self._ctx), "df_toggle_sky")
+        self._changed = True
+
+    def toggleSun(self):
+        self._check(self._lib.df_toggle_sun(self._ctx), "df_toggle_sun")
+        self._changed = True
+
+    def togglePrimitiveSelection(self, x, y):
+        self._check(self._lib.df_toggle_primitive_selection(self._ctx, int(x), int(y)), "df_toggle_primitive_selection")
+        self._check(self._lib.df_get_params(self._ctx, ctypes.byref(self._params)), "df_get_params")
+
+    def togglePrimitiveMaterial(self):
+        self._check(self._lib.df_toggle_primitive_material(self._ctx), "df_toggle_primitive_material")
+        self._changed = True
+
+    def getSelectedPrimitiveIndex(self):
+        o = self._params.selected_primitive_offset
+        return -1 if o == -1 else o // 3
+
+    def isChanged(self):
+        return self._changed
+
+    def setChanged(self, changed):
+        self._changed = bool(changed)

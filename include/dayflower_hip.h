@@ -1,0 +1,136 @@
+/*
+ * dayflower_hip.h - C ABI of the MI355X renderer (libdayflower_hip.so).
+ *
+ * Drop-in replacement for the reference's renderer-kernel surface:
+ * AbstractRendererKernel (src/main/java/org/dayflower/pathtracer/kernel/
+ * AbstractRendererKernel.java:36-727) plus the Aparapi Kernel calls the
+ * application makes on it (execute(Range), put, get, dispose;
+ * main/DayflowerApplication.java:319-343, 832-876).  Every entry point
+ * returns 0 on success or a negative DF_E* code (dayflower_types.h); the
+ * message of the last failure is available from df_last_error.  There is no
+ * CPU fallback: without a HIP device df_create fails with DF_ENODEVICE.
+ *
+ * Threading: a handle is not thread-safe; callers serialise calls on one
+ * handle, as the reference serialises execute/get/update with
+ * synchronized(pixels1) (DayflowerApplication.java:645-654, 852-860).
+ * Scalar parameters take effect at the next df_execute.
+ */
+#ifndef DAYFLOWER_HIP_H
+#define DAYFLOWER_HIP_H
+
+#include "dayflower_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct df_ctx df_ctx;
+
+/* Number of visible HIP devices (0 when none). */
+int df_device_count(int32_t* count);
+
+/* Creates a renderer on HIP device `device`.  Replaces
+ * new GPURendererKernel(sceneLoader) (GPURendererKernel.java:166-234) minus
+ * the scene, which df_load_scene supplies. */
+int df_create(df_ctx** out, int32_t device);
+
+/* Validates a compiled scene on the host without touching a device: every
+ * shape, surface, texture, triangle and BVH offset must lie inside its array
+ * and every BVH pointer must name a node further along the pre-order.  The
+ * same check guards df_load_scene.  Writes "ok" or the first problem to msg. */
+int df_validate_scene(const df_compiled_scene* scene, char* msg, size_t cap);
+
+/* Releases every device and host resource (Kernel.dispose). */
+void df_destroy(df_ctx* ctx);
+
+/* Uploads the compiled scene (the put() calls of GPURendererKernel.update,
+ * :426-447).  The arrays are validated (every offset in range), copied, and
+ * repacked device-side; the camera array is taken from scene->camera. */
+int df_load_scene(df_ctx* ctx, const df_compiled_scene* scene);
+
+/* Allocates the per-pixel film, sample counts, image, seed and primitive
+ * offset buffers for width x height and fills the primitive offsets with -1
+ * (update(W, H, byte[], localSize), GPURendererKernel.java:411-449 and
+ * AbstractImageKernel.java:145-165).  local_size_hint is accepted for
+ * signature parity and ignored (the workgroup shape is fixed at 16x16).
+ * Seeds are set from master seed 0; call df_seed or df_set_seeds after. */
+int df_update(df_ctx* ctx, int32_t width, int32_t height, int32_t local_size_hint);
+
+/* Fills the per-pixel 48-bit LCG seeds deterministically from a master
+ * seed: seed[i] = SplitMix64(master + (i + 1) * 0x9E3779B97F4A7C15).
+ * Replaces ThreadLocalRandom seeding (AbstractKernel.java:1148-1156). */
+int df_seed(df_ctx* ctx, uint64_t master);
+
+/* Sets / reads the per-pixel seeds exactly (n must equal width * height). */
+int df_set_seeds(df_ctx* ctx, const int64_t* seeds, size_t n);
+int df_get_seeds(df_ctx* ctx, int64_t* seeds, size_t n);
+
+/* updateCamera() (GPURendererKernel.java:455-461): re-uploads the 19-float
+ * camera array (Camera.update is host math, dfs_camera_setup). */
+int df_upload_camera(df_ctx* ctx, const float* camera, size_t n);
+
+/* updatePrimitives() (GPURendererKernel.java:467-521): new objectToWorld and
+ * worldToObject matrices, 16 floats per primitive, n_prim primitives. */
+int df_upload_transforms(df_ctx* ctx, const float* object_to_world, const float* world_to_object, int32_t n_prim);
+
+/* put(sceneSurfaces) after a material change (GPURendererKernel.java:314-342). */
+int df_upload_surfaces(df_ctx* ctx, const float* surfaces, size_t n);
+
+/* updateSunAndSky() (GPURendererKernel.java:527-569). */
+int df_upload_sky(df_ctx* ctx, const df_sky* sky);
+
+/* toggleSky() / toggleSun() (GPURendererKernel.java:374-397). */
+int df_toggle_sky(df_ctx* ctx);
+int df_toggle_sun(df_ctx* ctx);
+
+/* Renderer setters (AbstractRendererKernel.java:392-575).  Invalid enum
+ * values are rejected with DF_EINVAL (the reference ignores them). */
+int df_set_params(df_ctx* ctx, const df_params* params);
+int df_get_params(df_ctx* ctx, df_params* params);
+
+/* togglePrimitiveSelection(x, y) (GPURendererKernel.java:351-369): reads the
+ * primitive offset under pixel (x, y) and toggles the selection tint. */
+int df_toggle_primitive_selection(df_ctx* ctx, int32_t x, int32_t y);
+
+/* togglePrimitiveMaterial() (GPURendererKernel.java:314-342): cycles the
+ * material of the selected primitive's surface. */
+int df_toggle_primitive_material(df_ctx* ctx);
+
+/* clear() (AbstractImageKernel.java:119-121): the next pass overwrites the
+ * film instead of averaging into it. */
+int df_clear(df_ctx* ctx);
+
+/* execute(range) followed by clearFilmFlags(), `passes` times: one sample per
+ * pixel per pass.  Asynchronous on the handle's stream. */
+int df_execute(df_ctx* ctx, int32_t passes);
+
+/* Waits for the handle's stream. */
+int df_synchronize(df_ctx* ctx);
+
+/* Device time of the last df_execute in milliseconds (HIP events on the
+ * handle's stream; waits for it). */
+int df_last_execute_ms(df_ctx* ctx, float* ms);
+
+/* Readbacks (get(...)); sizes must match exactly. */
+int df_get_image_bgra(df_ctx* ctx, uint8_t* bgra, size_t n);        /* 4 * W * H bytes */
+int df_get_image_float(df_ctx* ctx, float* rgb, size_t n);          /* 3 * W * H */
+int df_get_primitive_offsets(df_ctx* ctx, int32_t* offsets, size_t n);  /* W * H */
+int df_get_film(df_ctx* ctx, float* rgb, size_t rgb_n, int32_t* samples, size_t samples_n);
+
+/* Restores film and sample counts (with df_set_seeds this resumes a render
+ * bit-exactly). */
+int df_set_film(df_ctx* ctx, const float* rgb, size_t rgb_n, const int32_t* samples, size_t samples_n);
+
+/* Cumulative work counters since the last reset (rays = intersection
+ * queries, samples = film updates). */
+int df_get_counters(df_ctx* ctx, df_counters* out);
+int df_reset_counters(df_ctx* ctx);
+
+/* Message of the last failed call on this handle (never NULL). */
+const char* df_last_error(df_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAYFLOWER_HIP_H */

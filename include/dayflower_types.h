@@ -1,0 +1,135 @@
+/*
+ * dayflower_types.h - plain-old-data types shared by the renderer C ABI
+ * (dayflower_hip.h) and the scene compiler C ABI (dayflower_scene.h).
+ *
+ * Every array keeps the flattened CompiledScene layout of the reference
+ * unchanged (src/main/java/org/dayflower/pathtracer/scene/compiler/CompiledScene.java:55-71,
+ * written in this order by CompiledScene.write :273-295).  Offsets stored in
+ * the arrays are element offsets exactly as the reference's SceneCompiler
+ * emits them (SceneCompiler.java:634-1053).
+ */
+#ifndef DAYFLOWER_TYPES_H
+#define DAYFLOWER_TYPES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: 0 = OK, negative = error.  There is no silent CPU fallback. */
+enum {
+  DF_OK = 0,
+  DF_EINVAL = -1,      /* bad argument / size mismatch                        */
+  DF_ENODEVICE = -2,   /* no HIP device or the requested ordinal is absent    */
+  DF_EHIP = -3,        /* a HIP runtime call failed (message in df_last_error) */
+  DF_ESTATE = -4,      /* call order violated (e.g. execute before update)    */
+  DF_ENOMEM = -5,      /* device or host allocation failed                    */
+  DF_EIO = -6          /* file read/write failed                               */
+};
+
+/* Renderer types (AbstractRendererKernel.java:40-52). */
+enum {
+  DF_RENDERER_AMBIENT_OCCLUSION = 1,
+  DF_RENDERER_PATH_TRACER = 2,
+  DF_RENDERER_RAY_CASTER = 3,
+  DF_RENDERER_RAY_MARCHER = 4,
+  DF_RENDERER_RAY_TRACER = 5,
+  DF_RENDERER_SURFACE_NORMALS = 6
+};
+
+/* Tone mappers (AbstractRendererKernel.java:58-64). */
+enum {
+  DF_TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED = 1,
+  DF_TONE_MAPPER_REINHARD = 2,
+  DF_TONE_MAPPER_REINHARD_MODIFIED_1 = 3,
+  DF_TONE_MAPPER_REINHARD_MODIFIED_2 = 4
+};
+
+/* Shader types (AbstractRendererKernel.java:54-56). */
+enum { DF_SHADER_FLAT = 1, DF_SHADER_GOURAUD = 2 };
+
+/* Transcendental-function rounding mode of the GPU kernels.
+ * DF_MATH_FAST  : single-precision device library (sinf, cosf, powf, ...).
+ * DF_MATH_EXACT : each transcendental evaluated in double and rounded once
+ *                 to float, i.e. the rounding of java.lang.Math under the
+ *                 reference's thread-pool execution.  Used for bit-level
+ *                 parity runs; every other operation is identical in both. */
+enum { DF_MATH_FAST = 0, DF_MATH_EXACT = 1 };
+
+/* The 14 flat arrays of a compiled scene plus the camera, as (pointer,
+ * element count) pairs, in CompiledScene file order.  The data is borrowed
+ * for the duration of the call that receives it and copied. */
+typedef struct df_compiled_scene {
+  const float* point2fs;        size_t point2fs_len;
+  const float* point3fs;        size_t point3fs_len;
+  const float* vector3fs;       size_t vector3fs_len;
+  const int32_t* planes;        size_t planes_len;
+  const float* spheres;         size_t spheres_len;
+  const float* terrains;        size_t terrains_len;
+  const int32_t* triangles;     size_t triangles_len;
+  const int32_t* bvh;           size_t bvh_len;
+  const float* textures;        size_t textures_len;
+  const float* surfaces;        size_t surfaces_len;
+  const int32_t* primitives;    size_t primitives_len;
+  const float* object_to_world; size_t object_to_world_len;
+  const float* world_to_object; size_t world_to_object_len;
+  const float* camera;          size_t camera_len;   /* 19 floats */
+} df_compiled_scene;
+
+/* Sun and sky scalars read by the kernel (GPURendererKernel.java:100-127,
+ * produced by Sky.set, Sky.java:295-374).  The sky frame is the reference's
+ * fixed ONB u=(1,0,0) v=(0,0,1) w=(0,1,0) unless a caller supplies another. */
+typedef struct df_sky {
+  float onb_u[3];
+  float onb_v[3];
+  float onb_w[3];
+  float sun_direction_world[3];
+  float sun_direction[3];       /* sun direction in the sky frame */
+  float sun_origin[3];          /* 10000 * sun_direction_world      */
+  float theta;                  /* acos(sun_direction.z)            */
+  float turbidity;              /* informational (host chain rule)  */
+  double zenith_relative_luminance;
+  double zenith_x;
+  double zenith_y;
+  double perez_relative_luminance[5];
+  double perez_x[5];
+  double perez_y[5];
+  int32_t is_sky_active;
+  int32_t is_sun_active;
+} df_sky;
+
+/* Renderer scalars (AbstractRendererKernel.java:129-184 defaults,
+ * AbstractKernel.java:196-205 global noise defaults). */
+typedef struct df_params {
+  int32_t renderer_type;
+  int32_t tone_mapper_type;
+  float tone_mapper_exposure;
+  int32_t pt_depth_maximum;
+  int32_t pt_depth_russian_roulette;
+  float ao_maximum_distance;
+  int32_t normal_mapping;
+  int32_t wireframes;
+  int32_t shader_type;
+  int32_t selected_primitive_offset;
+  float global_amplitude;
+  float global_frequency;
+  float global_gain;
+  float global_lacunarity;
+  int32_t global_octaves;
+  int32_t math_mode;
+} df_params;
+
+/* Work counters.  A ray is one intersection query (closest-hit or any-hit);
+ * a sample is one film update of one pixel. */
+typedef struct df_counters {
+  uint64_t rays;
+  uint64_t samples;
+} df_counters;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAYFLOWER_TYPES_H */

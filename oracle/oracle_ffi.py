@@ -1,0 +1,136 @@
+"""ctypes wrapper of liboracle.so - TEST INFRASTRUCTURE ONLY.
+
+The oracle is the CPU restatement of the reference renderer kernel
+(df_oracle.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg use this module, always as the checker or the reported CPU
+baseline; the product never imports it.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class OracleFrame(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("film", ctypes.c_void_p),
+        ("film_samples", ctypes.c_void_p),
+        ("image_float", ctypes.c_void_p),
+        ("image_bgra", ctypes.c_void_p),
+        ("primitive_offsets", ctypes.c_void_p),
+        ("seeds", ctypes.c_void_p),
+    ]
+
+
+class OracleCounters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("rays", "samples", "box_tests", "tri_tests", "prim_tests")]
+
+
+def build():
+    """Compiles liboracle.so from df_oracle.cpp (make)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (make -C oracle)")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.oracle_render.restype = ctypes.c_int
+        _lib.oracle_intersect.restype = ctypes.c_float
+        _lib.oracle_simplex2.restype = ctypes.c_float
+        _lib.oracle_simplex2.argtypes = [ctypes.c_float, ctypes.c_float]
+        _lib.oracle_simplex3.restype = ctypes.c_float
+        _lib.oracle_simplex3.argtypes = [ctypes.c_float] * 3
+        _lib.oracle_fbm3.restype = ctypes.c_float
+        _lib.oracle_fbm3.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        _lib.oracle_texture.restype = ctypes.c_int32
+        _lib.oracle_next_float.restype = ctypes.c_float
+        _lib.oracle_next_float.argtypes = [ctypes.POINTER(ctypes.c_int64)]
+    return _lib
+
+
+class OracleState:
+    """Film, seeds and outputs of one oracle render of width x height pixels."""
+
+    def __init__(self, width, height, seeds):
+        n = width * height
+        self.width, self.height = width, height
+        self.film = np.zeros(n * 3, np.float32)
+        self.film_samples = np.zeros(n, np.int32)
+        self.image_float = np.zeros(n * 3, np.float32)
+        self.image_bgra = np.zeros(n * 4, np.uint8)
+        self.primitive_offsets = np.full(n, -1, np.int32)
+        self.seeds = np.ascontiguousarray(seeds, np.int64).copy()
+        self.counters = {"rays": 0, "samples": 0, "box_tests": 0, "tri_tests": 0, "prim_tests": 0}
+
+    def frame(self):
+        return OracleFrame(self.width, self.height, self.film.ctypes.data, self.film_samples.ctypes.data,
+                           self.image_float.ctypes.data, self.image_bgra.ctypes.data,
+                           self.primitive_offsets.ctypes.data, self.seeds.ctypes.data)
+
+
+def render(scene, params, state, passes, clear_first=False, threads=None, pixel_begin=0, pixel_end=None, sky=None):
+    """Runs `passes` passes of run() over [pixel_begin, pixel_end)."""
+    if threads is None:
+        threads = os.cpu_count() or 1
+    if pixel_end is None:
+        pixel_end = state.width * state.height
+    cs = scene.as_struct()
+    fr = state.frame()
+    cnt = OracleCounters()
+    rc = lib().oracle_render(ctypes.byref(cs), ctypes.byref(scene.sky if sky is None else sky), ctypes.byref(params),
+                             ctypes.byref(fr), int(passes), 1 if clear_first else 0, int(threads),
+                             ctypes.c_int64(pixel_begin), ctypes.c_int64(pixel_end), ctypes.byref(cnt))
+    if rc != 0:
+        raise RuntimeError("oracle_render failed: %d" % rc)
+    for n, _ in OracleCounters._fields_:
+        state.counters[n] += getattr(cnt, n)
+    return state
+
+
+def intersect(scene, params, origin, direction, testing=False):
+    """One doIntersectPrimitives query: (t, 27-float record)."""
+    cs = scene.as_struct()
+    o = (ctypes.c_float * 3)(*origin)
+    d = (ctypes.c_float * 3)(*direction)
+    rec = (ctypes.c_float * 27)()
+    t = lib().oracle_intersect(ctypes.byref(cs), ctypes.byref(scene.sky), ctypes.byref(params), o, d,
+                               1 if testing else 0, rec)
+    return t, np.array(rec[:], np.float32)
+
+
+def sky_color(sky, direction, active=True):
+    d = (ctypes.c_float * 3)(*direction)
+    out = (ctypes.c_float * 3)()
+    lib().oracle_sky(ctypes.byref(sky), d, 1 if active else 0, out)
+    return np.array(out[:], np.float32)
+
+
+def texture_color(textures, offset, record):
+    t = np.ascontiguousarray(textures, np.float32)
+    rec = np.ascontiguousarray(record, np.float32)
+    return lib().oracle_texture(t.ctypes.data_as(_c_float_p), ctypes.c_size_t(t.size), int(offset),
+                                rec.ctypes.data_as(_c_float_p))
+
+
+def next_floats(seed, count):
+    """`count` draws of the per-pixel LCG from `seed`; returns (draws, final seed)."""
+    s = ctypes.c_int64(seed)
+    out = np.array([lib().oracle_next_float(ctypes.byref(s)) for _ in range(count)], np.float32)
+    return out, s.value

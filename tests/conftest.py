@@ -1,0 +1,58 @@
+import os
+import sys
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(REPO, "dayflower-path-tracer_amd")
+for p in (PKG_DIR, REPO):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
+
+
+def _ensure_built():
+    """Builds the libraries only when nothing is built yet."""
+    from dayflower_amd import _ffi
+
+    if not (os.path.exists(_ffi.HIP_LIB_PATH) and os.path.exists(_ffi.SCENE_LIB_PATH)):
+        import importlib.util
+
+        spec = importlib.util.spec_from_file_location("dayflower_build", os.path.join(PKG_DIR, "build.py"))
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        mod.build(jobs=4)
+    from oracle import oracle_ffi
+
+    if not os.path.exists(oracle_ffi.LIB_PATH):
+        oracle_ffi.build()
+
+
+_ensure_built()
+
+_scene_cache = {}
+
+
+@pytest.fixture(scope="session")
+def scenes():
+    """Workload scenes built once per session: scenes(config_id)."""
+    import dayflower_amd as dfa
+
+    def get(cid):
+        if cid not in _scene_cache:
+            _scene_cache[cid] = dfa.build_config(cid)
+        return _scene_cache[cid]
+
+    return get
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    import dayflower_amd as dfa
+
+    if dfa.device_count() < 1:
+        pytest.fail("no HIP device visible to a gpu-marked test")
+    return 0
