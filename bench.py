@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Throughput of the renderer on the 1920x1080 ~100k-triangle workload.
+
+A step is one pass of the path tracer over the full frame (one sample per
+pixel, GPURendererKernel.run() for every pixel) with inputs resident on the
+GPU.  `value` is Mrays/s over the timed steps, where a ray is one
+intersection query (closest-hit or any-hit), counted on the device.
+
+Besides the contract fields the JSON line carries:
+  roofline     - the render kernel's algorithmic FLOP rate (exact per-launch
+                 work counts x per-unit FLOP figures) over its HIP-event
+                 duration, against the FP32 peak; the byte figure is reported
+                 alongside;
+  cpu_baseline - the CPU oracle (the reference's per-pixel kernel restated in
+                 C++ with the reference's thread-pool semantics) on a band of
+                 rows of the same frame, on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "dayflower-path-tracer_amd"))
+sys.path.insert(0, REPO)
+
+METRIC = "Mrays/s on 100k-tri 1080p scene, batched vs plain; adjoint/forward time ratio"
+
+# Per-unit algorithmic work (hand-counted from the reference kernel lines):
+#   box test   (GPURendererKernel.java:1108-1134): 6 sub, 6 mul, 12 min/max;   24 B bounds + 8 B links
+#   triangle   (:1541-1594): ~45 FLOP incl. one division;                       36 B (9 floats)
+#   primitive  ray transform (:1085-1095): ~36 FLOP incl. 3 divisions;          48 B (12 floats)
+#   sphere     (:1433-1467): ~25 FLOP incl. sqrt and 2 divisions;               16 B
+#   terrain march step (:1484-1501): 2 sin + 12 FLOP, a sin counted as one op;  0 B
+#   sky        (:2127-2236): ~15 double transcendentals + ~45 FLOP;             0 B
+FLOP_PER = {"box_tests": 24, "tri_tests": 45, "prim_tests": 36, "sphere_tests": 25, "terrain_steps": 14,
+            "sky_evals": 60}
+BYTES_PER = {"box_tests": 32, "tri_tests": 36, "prim_tests": 48, "sphere_tests": 16}
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, params, seed, target_seconds, threads):
+    """The oracle on a band of rows of one pass of the same frame."""
+    import numpy as np
+
+    import dayflower_amd as dfa
+    from oracle import oracle_ffi
+
+    w, h = scene.width, scene.height
+    seeds = dfa.pixel_seeds(seed, w * h)
+    start_row = h // 2 - 8
+    st = oracle_ffi.OracleState(w, h, seeds)
+    t0 = time.perf_counter()
+    oracle_ffi.render(scene, params, st, 1, threads=threads, pixel_begin=start_row * w, pixel_end=(start_row + 4) * w)
+    probe = max(time.perf_counter() - t0, 1e-3)
+    rows = int(max(4, min(h, 4 * target_seconds / probe)))
+    start_row = max(0, h // 2 - rows // 2)
+    rows = min(rows, h - start_row)
+    st = oracle_ffi.OracleState(w, h, seeds)
+    t0 = time.perf_counter()
+    oracle_ffi.render(scene, params, st, 1, threads=threads, pixel_begin=start_row * w,
+                      pixel_end=(start_row + rows) * w)
+    dt = time.perf_counter() - t0
+    rays = st.counters["rays"]
+    return {
+        "value": round(rays / dt / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "rows %d-%d (%d pixels) of one 1920x1080 pass, same seeds and parameters: %d rays in %.2f s"
+                  % (start_row, start_row + rows - 1, rows * w, rays, dt),
+    }, np.float64(dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=3, help="benchmark workload id (3 = 1920x1080 mesh + terrain)")
+    ap.add_argument("--math", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import dayflower_amd as dfa
+
+    if not torch.cuda.is_available():
+        log("no HIP device visible")
+        sys.exit(1)
+
+    scene = dfa.build_config(args.workload)
+    params = scene.copy_params()
+    params.math_mode = dfa.MATH_FAST if args.math == "fast" else dfa.MATH_EXACT
+    stats = dfa.scene.stats(scene)
+    seed = 42
+    k = dfa.HIPRendererKernel(scene, device=0)
+    k.setParams(params)
+    k.update(scene.width, scene.height)
+    k.updateCamera()
+    k.seed(seed)
+    for _ in range(args.warmup):
+        k.execute(1)
+    k.synchronize()
+    k.resetCounters()
+
+    kernel_ms = 0.0
+    torch.cuda.synchronize()
+    k.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        k.execute(1)
+        kernel_ms += k.lastExecuteMillis()
+    k.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    c = k.counters()
+    k.dispose()
+
+    wall = t1 - t0
+    steps = args.steps
+    value = c["rays"] / wall / 1e6
+    avg_kernel_s = kernel_ms / steps / 1e3
+    flop = sum(FLOP_PER[n] * c[n] for n in FLOP_PER) / steps
+    alg_bytes = sum(BYTES_PER[n] * c[n] for n in BYTES_PER) / steps
+    achieved_tflops = flop / avg_kernel_s / 1e12
+    achieved_gbs = alg_bytes / avg_kernel_s / 1e9
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved_tflops, 4),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 6),
+        "traffic": None,
+        "kernel": "render_kernel<0>",
+        "note": "VALU-bound (no matrix work); peak is the FP32 figure; algorithmic bytes %.3g GB/s = %.4f of HBM"
+                % (achieved_gbs, achieved_gbs / PEAK_HBM_GBS),
+        "avg_kernel_ms": round(avg_kernel_s * 1e3, 3),
+        "flop_per_launch": flop,
+        "bytes_per_launch": alg_bytes,
+    }
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        cpu, _ = cpu_baseline(scene, params, seed, args.cpu_seconds, threads)
+        cpu["gpu_over_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: procedural workload %d (%s), seeds documented in the scene compiler" % (args.workload,
+                                                                                                  scene.name),
+        "config": {
+            "workload": "%s: %dx%d, 1 sample per pixel per step, depth %d, %d triangles, %d spheres, %d terrain, "
+                        "%s math" % (scene.name, scene.width, scene.height, params.pt_depth_maximum,
+                                     stats["triangles"], stats["spheres"], stats["terrains"], args.math),
+            "samples_per_step": scene.width * scene.height,
+            "rays_per_step": c["rays"] / steps,
+            "samples_per_s": round(c["samples"] / wall, 1),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

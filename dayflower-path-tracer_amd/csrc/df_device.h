@@ -355,13 +355,16 @@ __device__ __forceinline__ float isect_triangle(float ox, float oy, float oz, fl
 }
 
 template <int M>
-__device__ float isect_terrain(float ox, float oy, float oz, float dx, float dy, float dz) {  // :1470-1506
+__device__ float isect_terrain(float ox, float oy, float oz, float dx, float dy, float dz,
+                               unsigned* steps) {  // :1470-1506
   const float sr = 1.0f / 10.0f;
   float t = 0.0f;
   float td = 0.01f;
   float lh = 0.0f, ly = 0.0f;
   const float bx = ox * sr, by = oy * sr, bz = oz * sr;
+  unsigned n = 0;
   for (float tc = 0.001f; tc < 10.0f; tc += td) {
+    ++n;
     const float px = bx + dx * tc;
     const float py = by + dy * tc;
     const float pz = bz + dz * tc;
@@ -374,6 +377,7 @@ __device__ float isect_terrain(float ox, float oy, float oz, float dx, float dy,
     lh = h;
     ly = py;
   }
+  *steps += n;
   return t > DF_EPS ? t : DF_INF;
 }
 

@@ -126,7 +126,8 @@ struct DFrame {
   uint8_t* image_bgra;     // 4 * W * H
   int32_t* primitive_offsets;
   uint64_t* seeds;
-  unsigned long long* counters;  // [rays, samples]
+  unsigned long long* counters;  // df_counters order: rays, samples, box, tri, prim, sphere, terrain steps, sky
+
 };
 
 }  // namespace dfk

@@ -73,6 +73,8 @@ __device__ void image_pass(const DParams& P, const float film[3], int prim_off, 
   *reinterpret_cast<uint32_t*>(out_b) = px;
 }
 
+constexpr int kCounterCount = 8;
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K
   __syncthreads();
   const int x = blockIdx.x * 16 + threadIdx.x;
   const int y = blockIdx.y * 16 + threadIdx.y;
-  unsigned long long rays = 0, samples = 0;
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (x < P.width && y < P.height) {
     const int gid = y * P.width + x;
     Tracer<M> tr(S, C, K, P, perm);
@@ -151,14 +153,19 @@ __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K
     F.film_samples[gid] = nsamp;
     F.primitive_offsets[gid] = prim_off;
     image_pass<M>(P, film, prim_off, F.image_float + gid * 3, F.image_bgra + gid * 4);
-    rays = tr.rays;
-    samples = tr.samples;
+    cnt[0] = tr.rays;
+    cnt[1] = tr.samples;
+    cnt[2] = tr.box_tests;
+    cnt[3] = tr.tri_tests;
+    cnt[4] = tr.prim_tests;
+    cnt[5] = tr.sphere_tests;
+    cnt[6] = tr.terrain_steps;
+    cnt[7] = tr.sky_evals;
   }
-  rays = wave_sum(rays);
-  samples = wave_sum(samples);
-  if ((threadIdx.x + threadIdx.y * 16) % 64 == 0) {
-    atomicAdd(&F.counters[0], rays);
-    atomicAdd(&F.counters[1], samples);
+  const bool leader = (threadIdx.x + threadIdx.y * 16) % 64 == 0;
+  for (int i = 0; i < kCounterCount; ++i) {
+    const unsigned long long v = wave_sum(cnt[i]);
+    if (leader && v) atomicAdd(&F.counters[i], v);
   }
 }
 

@@ -571,8 +571,8 @@ int df_create(df_ctx** out, int32_t device) {
     df_destroy(c);
     return DF_EHIP;
   }
-  if (alloc_buf(c, c->counters, 2 * sizeof(unsigned long long)) != DF_OK ||
-      hipMemset(c->counters.p, 0, 2 * sizeof(unsigned long long)) != hipSuccess) {
+  if (alloc_buf(c, c->counters, sizeof(df_counters)) != DF_OK ||
+      hipMemset(c->counters.p, 0, sizeof(df_counters)) != hipSuccess) {
     df_destroy(c);
     return DF_ENOMEM;
   }
@@ -865,18 +865,15 @@ int df_set_film(df_ctx* c, const float* rgb, size_t rgb_n, const int32_t* sample
 
 int df_get_counters(df_ctx* c, df_counters* out) {
   if (!c || !out) return DF_EINVAL;
-  unsigned long long v[2] = {0, 0};
   DF_HIP(c, hipStreamSynchronize(c->stream));
-  DF_HIP(c, hipMemcpy(v, c->counters.p, sizeof v, hipMemcpyDeviceToHost));
-  out->rays = v[0];
-  out->samples = v[1];
+  DF_HIP(c, hipMemcpy(out, c->counters.p, sizeof(df_counters), hipMemcpyDeviceToHost));
   return DF_OK;
 }
 
 int df_reset_counters(df_ctx* c) {
   if (!c) return DF_EINVAL;
   DF_HIP(c, hipStreamSynchronize(c->stream));
-  DF_HIP(c, hipMemset(c->counters.p, 0, 2 * sizeof(unsigned long long)));
+  DF_HIP(c, hipMemset(c->counters.p, 0, sizeof(df_counters)));
   return DF_OK;
 }
 

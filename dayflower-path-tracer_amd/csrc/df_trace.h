@@ -24,9 +24,16 @@ struct Tracer {
   float tmp[3];
   unsigned rays;
   unsigned samples;
+  unsigned box_tests;
+  unsigned tri_tests;
+  unsigned prim_tests;
+  unsigned sphere_tests;
+  unsigned terrain_steps;
+  unsigned sky_evals;
 
   __device__ Tracer(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const int* perm)
-      : S(s), C(c), K(k), P(p), rays(0), samples(0) {
+      : S(s), C(c), K(k), P(p), rays(0), samples(0), box_tests(0), tri_tests(0), prim_tests(0),
+        sphere_tests(0), terrain_steps(0), sky_evals(0) {
     noise.perm = perm;
     h.t = DF_INF;
     h.prim = -1;
@@ -105,6 +112,7 @@ struct Tracer {
     for (int i = 0; i < n; ++i) {
       const DPrim& pr = S.prims[i];
       const int st = pr.type;
+      ++prim_tests;
       const float* m = pr.w2o;
       const float oxo = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
       const float oyo = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
@@ -118,6 +126,7 @@ struct Tracer {
         int idx = pr.node_root;
         while (idx != -1) {
           const DNode nd = S.nodes[idx];
+          ++box_tests;
           const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
           const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
           const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
@@ -134,6 +143,7 @@ struct Tracer {
             int k = nd.tri;
             for (;;) {
               const DTri tr = S.tris[k];
+              ++tri_tests;
               cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
               if (cur < closest) {
                 closest = cur;
@@ -152,11 +162,13 @@ struct Tracer {
         cur = isect_plane(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], pr.geo[4],
                           pr.geo[5], closest);
       } else if (st == 2) {
+        ++sphere_tests;
         cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
       } else if (st == 3) {
-        cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo);
+        cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, &terrain_steps);
       } else if (st == 4) {
         const DTri tr = S.tris[pr.node_root];
+        ++tri_tests;
         cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
       }
       if (cur < closest) {
@@ -580,6 +592,7 @@ struct Tracer {
     float d0Z = dx * K.onb_w[0] + dy * K.onb_w[1] + dz * K.onb_w[2];
     float r = 0.01f, g = 0.01f, b = 0.01f;
     if (d0Z >= 0.0f && active) {
+      ++sky_evals;
       d0Z = jmax(d0Z, 0.001f);
       const float lr = jrsqrt(d0X * d0X + d0Y * d0Y + d0Z * d0Z);
       const float d1X = d0X * lr, d1Y = d0Y * lr, d1Z = d0Z * lr;

@@ -93,8 +93,12 @@ class Params(ctypes.Structure):
     ]
 
 
+COUNTER_NAMES = ("rays", "samples", "box_tests", "tri_tests", "prim_tests", "sphere_tests", "terrain_steps",
+                 "sky_evals")
+
+
 class Counters(ctypes.Structure):
-    _fields_ = [("rays", ctypes.c_uint64), ("samples", ctypes.c_uint64)]
+    _fields_ = [(n, ctypes.c_uint64) for n in COUNTER_NAMES]
 
 
 class SceneStats(ctypes.Structure):

@@ -223,7 +223,7 @@ class HIPRendererKernel:
     def counters(self):
         c = _ffi.Counters()
         self._check(self._lib.df_get_counters(self._ctx, ctypes.byref(c)), "df_get_counters")
-        return {"rays": c.rays, "samples": c.samples}
+        return {n: getattr(c, n) for n in _ffi.COUNTER_NAMES}
 
     def resetCounters(self):
         self._check(self._lib.df_reset_counters(self._ctx), "df_reset_counters")

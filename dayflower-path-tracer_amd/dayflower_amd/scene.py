@@ -79,6 +79,12 @@ class CompiledScene:
             setattr(cs, n + "_len", a.size)
         return cs
 
+    def copy_params(self):
+        """An independent copy of the workload's renderer parameters."""
+        p = _ffi.Params()
+        ctypes.memmove(ctypes.byref(p), ctypes.byref(self.params), ctypes.sizeof(_ffi.Params))
+        return p
+
     def validate(self):
         """df_validate_scene: (status, message) of the host-side range checks."""
         buf = ctypes.create_string_buffer(512)
@@ -88,7 +94,8 @@ class CompiledScene:
     def with_camera(self, camera):
         arrays = dict(self.arrays)
         arrays["camera"] = np.ascontiguousarray(camera, np.float32)
-        return CompiledScene(self.name, arrays, self.sky, int(camera[16]), int(camera[17]), self.spp, self.params)
+        return CompiledScene(self.name, arrays, self.sky, int(camera[16]), int(camera[17]), self.spp,
+                             self.copy_params())
 
     def write(self, path):
         """CompiledScene.write (CompiledScene.java:273-320)."""

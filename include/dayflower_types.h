@@ -122,10 +122,19 @@ typedef struct df_params {
 } df_params;
 
 /* Work counters.  A ray is one intersection query (closest-hit or any-hit);
- * a sample is one film update of one pixel. */
+ * a sample is one film update of one pixel.  The remaining fields count the
+ * algorithmic units of the intersection and sky work: BVH box tests,
+ * triangle tests, primitive ray transforms, sphere tests, terrain march
+ * steps and sky evaluations (the Perez model of an escaped ray). */
 typedef struct df_counters {
   uint64_t rays;
   uint64_t samples;
+  uint64_t box_tests;
+  uint64_t tri_tests;
+  uint64_t prim_tests;
+  uint64_t sphere_tests;
+  uint64_t terrain_steps;
+  uint64_t sky_evals;
 } df_counters;
 
 #ifdef __cplusplus

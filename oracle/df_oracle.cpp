@@ -153,7 +153,8 @@ void daylight_xyy_to_srgb(float fx, float fy, double lum, float* r_out, float* g
 }
 
 struct Counters {
-  uint64_t rays = 0, samples = 0, box_tests = 0, tri_tests = 0, prim_tests = 0;
+  uint64_t rays = 0, samples = 0, box_tests = 0, tri_tests = 0, prim_tests = 0, sphere_tests = 0, terrain_steps = 0,
+           sky_evals = 0;
 };
 
 struct Scene {
@@ -303,7 +304,7 @@ class Item {
                                float az, float nx, float ny, float nz, float tmin, float tmax);
   static float intersect_sphere(float ox, float oy, float oz, float dx, float dy, float dz, float px, float py,
                                 float pz, float r, float tmin, float tmax);
-  static float intersect_terrain(float ox, float oy, float oz, float dx, float dy, float dz);
+  static float intersect_terrain(float ox, float oy, float oz, float dx, float dy, float dz, uint64_t* steps);
   static float intersect_triangle(float ox, float oy, float oz, float dx, float dy, float dz, float ax, float ay,
                                   float az, float bx, float by, float bz, float cx, float cy, float cz, float tmin,
                                   float tmax);
@@ -528,7 +529,8 @@ float Item::intersect_sphere(float ox, float oy, float oz, float dx, float dy, f
   return INFINITY_;
 }
 
-float Item::intersect_terrain(float ox, float oy, float oz, float dx, float dy, float dz) {  // :1470-1506
+float Item::intersect_terrain(float ox, float oy, float oz, float dx, float dy, float dz,
+                             uint64_t* steps) {  // :1470-1506
   const float scale = 10.0f;
   const float sr = 1.0f / scale;
   float t = 0.0f;
@@ -536,6 +538,7 @@ float Item::intersect_terrain(float ox, float oy, float oz, float dx, float dy, 
   float td = 0.01f;
   float lh = 0.0f, ly = 0.0f;
   for (float tc = tmin; tc < tmax; tc += td) {
+    ++*steps;
     const float px = ox * sr + dx * tc;
     const float py = oy * sr + dy * tc;
     const float pz = oz * sr + dz * tc;
@@ -648,11 +651,12 @@ float Item::do_intersect_primitives(float ox, float oy, float oz, float dx, floa
       const float* n = v3() + pl[3];
       cur = intersect_plane(oxo, oyo, ozo, dxo, dyo, dzo, a[0], a[1], a[2], n[0], n[1], n[2], 0.001f, closest);
     } else if (st == 2) {
+      cnt_.sphere_tests++;
       const float* pos = p3() + jf2i(A->spheres[so + 0]);
       cur = intersect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pos[0], pos[1], pos[2], A->spheres[so + 1], 0.001f,
                              closest);
     } else if (st == 3) {
-      cur = intersect_terrain(oxo, oyo, ozo, dxo, dyo, dzo);
+      cur = intersect_terrain(oxo, oyo, ozo, dxo, dyo, dzo, &cnt_.terrain_steps);
     } else if (st == 4) {
       const float* a = p3() + A->triangles[so + 0];
       const float* b = p3() + A->triangles[so + 1];
@@ -1152,6 +1156,7 @@ void Item::do_sky(bool active, float dx, float dy, float dz) {  // :2127-2236
   float d0Z = dx * S->onb_w[0] + dy * S->onb_w[1] + dz * S->onb_w[2];
   float r = 0.01f, g = 0.01f, b = 0.01f;
   if (d0Z >= 0.0f && active) {
+    cnt_.sky_evals++;
     d0Z = jmaxf(d0Z, 0.001f);
     const float lr = jrsqrt(d0X * d0X + d0Y * d0Y + d0Z * d0Z);
     const float d1X = d0X * lr, d1Y = d0Y * lr, d1Z = d0Z * lr;
@@ -1812,7 +1817,7 @@ typedef struct oracle_frame {
 } oracle_frame;
 
 typedef struct oracle_counters {
-  uint64_t rays, samples, box_tests, tri_tests, prim_tests;
+  uint64_t rays, samples, box_tests, tri_tests, prim_tests, sphere_tests, terrain_steps, sky_evals;
 } oracle_counters;
 
 // Runs `passes` passes of run() over pixels [pixel_begin, pixel_end) with
@@ -1852,6 +1857,9 @@ int oracle_render(const df_compiled_scene* scene, const df_sky* sky, const df_pa
       counters->box_tests += c.box_tests;
       counters->tri_tests += c.tri_tests;
       counters->prim_tests += c.prim_tests;
+      counters->sphere_tests += c.sphere_tests;
+      counters->terrain_steps += c.terrain_steps;
+      counters->sky_evals += c.sky_evals;
     }
   }
   return 0;

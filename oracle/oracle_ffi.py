@@ -32,7 +32,8 @@ class OracleFrame(ctypes.Structure):
 
 
 class OracleCounters(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_uint64) for n in ("rays", "samples", "box_tests", "tri_tests", "prim_tests")]
+    _fields_ = [(n, ctypes.c_uint64) for n in ("rays", "samples", "box_tests", "tri_tests", "prim_tests",
+                                               "sphere_tests", "terrain_steps", "sky_evals")]
 
 
 def build():
@@ -77,7 +78,7 @@ class OracleState:
         self.image_bgra = np.zeros(n * 4, np.uint8)
         self.primitive_offsets = np.full(n, -1, np.int32)
         self.seeds = np.ascontiguousarray(seeds, np.int64).copy()
-        self.counters = {"rays": 0, "samples": 0, "box_tests": 0, "tri_tests": 0, "prim_tests": 0}
+        self.counters = {n: 0 for n, _ in OracleCounters._fields_}
 
     def frame(self):
         return OracleFrame(self.width, self.height, self.film.ctypes.data, self.film_samples.ctypes.data,
@@ -88,7 +89,7 @@ class OracleState:
 def render(scene, params, state, passes, clear_first=False, threads=None, pixel_begin=0, pixel_end=None, sky=None):
     """Runs `passes` passes of run() over [pixel_begin, pixel_end)."""
     if threads is None:
-        threads = os.cpu_count() or 1
+        threads = min(16, os.cpu_count() or 1)
     if pixel_end is None:
         pixel_end = state.width * state.height
     cs = scene.as_struct()
