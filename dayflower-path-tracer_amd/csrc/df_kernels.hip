@@ -80,7 +80,10 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-template <int M>
+// M: transcendental mode; R: renderer type (1 AO, 2 path tracer, 3 ray caster,
+// 4 ray marcher, 5 ray tracer, 6 surface normals), a template parameter so
+// that each instantiation holds only its own renderer's ray queries.
+template <int M, int R>
 __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K, DParams P, DFrame F) {
   __shared__ int perm[512];
   for (int i = threadIdx.y * 16 + threadIdx.x; i < 512; i += 256) perm[i] = kPerm[i & 255];
@@ -103,25 +106,18 @@ __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K
       float col[3] = {0.0f, 0.0f, 0.0f};
       bool set = false;  // filmSetColor instead of filmAddColor
       if (tr.primary_ray(x, y, ray)) {
-        switch (P.renderer_type) {
-          case 1:
-            tr.render_ao(ray, col, &prim_off);
-            break;
-          case 3:
-            tr.render_rc(ray, sky_active, col, &prim_off);
-            break;
-          case 4:
-            tr.render_rm(ray, sky_active, col);
-            break;
-          case 5:
-            tr.render_rt(ray, sky_active, col, &prim_off);
-            break;
-          case 6:
-            tr.render_sn(ray, sky_active, col, &prim_off);
-            break;
-          default:
-            tr.render_pt(ray, sky_active, sun_active, col, &prim_off);
-            break;
+        if constexpr (R == 1) {
+          tr.render_ao(ray, col, &prim_off);
+        } else if constexpr (R == 3) {
+          tr.render_rc(ray, sky_active, col, &prim_off);
+        } else if constexpr (R == 4) {
+          tr.render_rm(ray, sky_active, col);
+        } else if constexpr (R == 5) {
+          tr.render_rt(ray, sky_active, col, &prim_off);
+        } else if constexpr (R == 6) {
+          tr.render_sn(ray, sky_active, col, &prim_off);
+        } else {
+          tr.render_pt(ray, sky_active, sun_active, col, &prim_off);
         }
       } else {
         set = true;
@@ -189,11 +185,13 @@ hipError_t dfk_launch_render(const dfk::DScene& s, const dfk::DCamera& c, const 
                              const dfk::DFrame& f, int math_mode, hipStream_t stream) {
   const dim3 block(16, 16);
   const dim3 grid((p.width + 15) / 16, (p.height + 15) / 16);
-  if (math_mode == 1) {
-    hipLaunchKernelGGL(dfk::render_kernel<1>, grid, block, 0, stream, s, c, k, p, f);
-  } else {
-    hipLaunchKernelGGL(dfk::render_kernel<0>, grid, block, 0, stream, s, c, k, p, f);
-  }
+  using Fn = void (*)(dfk::DScene, dfk::DCamera, dfk::DSky, dfk::DParams, dfk::DFrame);
+  static const Fn kFast[6] = {dfk::render_kernel<0, 1>, dfk::render_kernel<0, 2>, dfk::render_kernel<0, 3>,
+                              dfk::render_kernel<0, 4>, dfk::render_kernel<0, 5>, dfk::render_kernel<0, 6>};
+  static const Fn kExact[6] = {dfk::render_kernel<1, 1>, dfk::render_kernel<1, 2>, dfk::render_kernel<1, 3>,
+                               dfk::render_kernel<1, 4>, dfk::render_kernel<1, 5>, dfk::render_kernel<1, 6>};
+  const int r = p.renderer_type >= 1 && p.renderer_type <= 6 ? p.renderer_type - 1 : 1;
+  hipLaunchKernelGGL((math_mode == 1 ? kExact : kFast)[r], grid, block, 0, stream, s, c, k, p, f);
   return hipGetLastError();
 }
 

@@ -46,7 +46,7 @@ struct Tracer {
   }
 
   // ---------------------------------------------------------- primary ray (:573-724)
-  __device__ bool primary_ray(int x, int y, float ray[6]) {
+  __device__ __forceinline__ bool primary_ray(int x, int y, float ray[6]) {
     const float* c = C.c;
     const float eyeX = c[1], eyeY = c[2], eyeZ = c[3];
     const float uX = c[7], uY = c[8], uZ = c[9];
@@ -104,7 +104,14 @@ struct Tracer {
   }
 
   // ---------------------------------------------------------- ray query (:1048-1411)
-  __device__ float intersect(float ox, float oy, float oz, float dx, float dy, float dz, bool testing) {
+  // Testing = true is the any-hit query (isTesting), false the closest-hit
+  // query that also fills the hit record.
+  __device__ __forceinline__ float intersect(float ox, float oy, float oz, float dx, float dy, float dz, bool testing) {
+    return testing ? intersect_t<true>(ox, oy, oz, dx, dy, dz) : intersect_t<false>(ox, oy, oz, dx, dy, dz);
+  }
+
+  template <bool testing>
+  __device__ __forceinline__ float intersect_t(float ox, float oy, float oz, float dx, float dy, float dz) {
     ++rays;
     float closest = DF_INF;
     int ci = -1, cst = -1, cso = -1;
@@ -217,7 +224,7 @@ struct Tracer {
 
   // ---------------------------------------------------------- surface properties
 
-  __device__ void props_plane(float ox, float oy, float oz, float dx, float dy, float dz, float t,
+  __device__ __forceinline__ void props_plane(float ox, float oy, float oz, float dx, float dy, float dz, float t,
                               int so) {  // :2349-2450 (UV part; the plane ONB is never read)
     const int* pl = S.planes + so;
     const float* A = S.point3fs + pl[0];
@@ -252,7 +259,7 @@ struct Tracer {
     h.ns[2] = nZ;
   }
 
-  __device__ void props_sphere(float ox, float oy, float oz, float dx, float dy, float dz, float t,
+  __device__ __forceinline__ void props_sphere(float ox, float oy, float oz, float dx, float dy, float dz, float t,
                                const DPrim& pr) {  // :2452-2492
     const float pX = ox + dx * t, pY = oy + dy * t, pZ = oz + dz * t;
     const float n0X = pX - pr.geo[0], n0Y = pY - pr.geo[1], n0Z = pZ - pr.geo[2];
@@ -268,7 +275,7 @@ struct Tracer {
     h.ns[2] = nZ;
   }
 
-  __device__ void props_terrain(float ox, float oy, float oz, float dx, float dy, float dz, float t) {  // :2495-2539
+  __device__ __forceinline__ void props_terrain(float ox, float oy, float oz, float dx, float dy, float dz, float t) {  // :2495-2539
     const float sr = 1.0f / 10.0f;
     const float pX = ox * sr + dx * t, pY = oy * sr + dy * t, pZ = oz * sr + dz * t;
     const float e = 0.02f;
@@ -286,7 +293,7 @@ struct Tracer {
     h.ns[2] = nZ * r;
   }
 
-  __device__ void props_triangle(float ox, float oy, float oz, float dx, float dy, float dz, float t,
+  __device__ __forceinline__ void props_triangle(float ox, float oy, float oz, float dx, float dy, float dz, float t,
                                  int to) {  // :2541-2650
     const int32_t* T = S.triangles + to;
     const float* A = S.point3fs + T[0];
@@ -341,7 +348,7 @@ struct Tracer {
     h.tan[2] = t0Z * tr;
   }
 
-  __device__ void to_world(const DPrim& pr) {  // :3998-4073 (the fields that are read)
+  __device__ __forceinline__ void to_world(const DPrim& pr) {  // :3998-4073 (the fields that are read)
     const float* w = pr.w2o;
     const float* o = pr.o2w;
     const float x = h.p[0], y = h.p[1], z = h.p[2];
@@ -358,7 +365,7 @@ struct Tracer {
     h.tan[2] = w[2] * tx + w[6] * ty + w[10] * tz;
   }
 
-  __device__ void normal_map_noise(const DPrim& pr) {  // :2658-2712
+  __device__ __forceinline__ void normal_map_noise(const DPrim& pr) {  // :2658-2712
     const int so = pr.surface_offset;
     const float amount = S.surfaces[so + 4];
     const float scale = S.surfaces[so + 5];
@@ -378,7 +385,7 @@ struct Tracer {
     }
   }
 
-  __device__ void normal_map_image(const DPrim& pr) {  // :2714-2846
+  __device__ __forceinline__ void normal_map_image(const DPrim& pr) {  // :2714-2846
     const int so = pr.surface_offset;
     const int to = jf2i(S.surfaces[so + 3]);
     const int tt = jf2i(S.textures[to]);
@@ -433,7 +440,7 @@ struct Tracer {
 
   // ---------------------------------------------------------- textures (:1596-1960)
 
-  __device__ int tex_color(int off) {
+  __device__ __forceinline__ int tex_color(int off) {
     const int type = jf2i(S.textures[off]);
     if (type == 1) {
       const float* T = S.textures;
@@ -448,7 +455,7 @@ struct Tracer {
     return tex_color2(off);
   }
 
-  __device__ int tex_color2(int off) {
+  __device__ __forceinline__ int tex_color2(int off) {
     const float* T = S.textures;
     const int type = jf2i(T[off]);
     switch (type) {
@@ -507,7 +514,7 @@ struct Tracer {
     }
   }
 
-  __device__ void image_coords(int off, float* u2, float* v2, int* x, int* y, int* w, int* res) {
+  __device__ __forceinline__ void image_coords(int off, float* u2, float* v2, int* x, int* y, int* w, int* res) {
     const float* T = S.textures;
     const float u = h.uv[0], v = h.uv[1];
     const float width = T[off + 4], height = T[off + 5];
@@ -524,7 +531,7 @@ struct Tracer {
     *res = jf2i(width * height);
   }
 
-  __device__ int tex_image(int off) {  // :1817-1892
+  __device__ __forceinline__ int tex_image(int off) {  // :1817-1892
     float u2, v2;
     int x, y, w, res;
     image_coords(off, &u2, &v2, &x, &y, &w, &res);
@@ -542,7 +549,7 @@ struct Tracer {
     return quant3(r, g, b);
   }
 
-  __device__ int tex_image_simple(int off) {  // :1894-1929
+  __device__ __forceinline__ int tex_image_simple(int off) {  // :1894-1929
     float u2, v2;
     int x, y, w, res;
     image_coords(off, &u2, &v2, &x, &y, &w, &res);
@@ -552,7 +559,7 @@ struct Tracer {
 
   // ---------------------------------------------------------- lighting
 
-  __device__ int phong_shader0(bool check, float pX, float pY, float pZ, float nX, float nY, float nZ, float vX,
+  __device__ __forceinline__ int phong_shader0(bool check, float pX, float pY, float pZ, float nX, float nY, float nZ, float vX,
                                float vY, float vZ, float aR, float aG, float aB) {  // :1962-2039 (ka .2, kd .5, ks .5, ns 250)
     float r = 0.0f, g = 0.0f, b = 0.0f;
     r += aR * 0.2f * 1.0f;
@@ -562,7 +569,7 @@ struct Tracer {
     const float l2 = lX * lX + lY * lY + lZ * lZ;
     const float lr = 1.0f / sqrtf(l2);
     const float lnX = lX * lr, lnY = lY * lr, lnZ = lZ * lr;
-    if (check) intersect(pX, pY, pZ, lnX, lnY, lnZ, false);
+    if (check) intersect_t<false>(pX, pY, pZ, lnX, lnY, lnZ);
     const float t = check ? h.t : DF_INF;
     if (t >= DF_INF || t > l2) {
       const float ldn = lnX * nX + lnY * nY + lnZ * nZ;
@@ -586,7 +593,7 @@ struct Tracer {
            (jf2i(b * 255.0f + 0.5f) & 0xFF);
   }
 
-  __device__ void sky(bool active, float dx, float dy, float dz) {  // :2127-2236
+  __device__ __forceinline__ void sky(bool active, float dx, float dy, float dz) {  // :2127-2236
     float d0X = dx * K.onb_u[0] + dy * K.onb_u[1] + dz * K.onb_u[2];
     float d0Y = dx * K.onb_v[0] + dy * K.onb_v[1] + dz * K.onb_v[2];
     float d0Z = dx * K.onb_w[0] + dy * K.onb_w[1] + dz * K.onb_w[2];
@@ -637,7 +644,7 @@ struct Tracer {
     tmp[2] = b;
   }
 
-  __device__ void sun(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ, float aR, float aG,
+  __device__ __forceinline__ void sun(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ, float aR, float aG,
                       float aB) {  // :2238-2347
     float r = 0.0f, g = 0.0f, b = 0.0f;
     if (active) {
@@ -671,7 +678,7 @@ struct Tracer {
       const float dp0 = dX * sdX + dY * sdY + dZ * sdZ;
       const float dp1 = dX * nX + dY * nY + dZ * nZ;
       if (dp0 > 0.0f && dp1 > 0.0f) {
-        const float t = intersect(pX, pY, pZ, dX, dY, dZ, true);
+        const float t = intersect_t<true>(pX, pY, pZ, dX, dY, dZ);
         if (t >= DF_INF - 0.0001f) {
           r = aR * 1.0f * dp1 * DF_PI_RECIP;
           g = aG * 1.0f * dp1 * DF_PI_RECIP;
@@ -685,7 +692,7 @@ struct Tracer {
   }
 
   // Cosine-weighted direction about w (:3410-3444, :3466-3500).
-  __device__ void cosine_dir(float wX, float wY, float wZ, float* o) {
+  __device__ __forceinline__ void cosine_dir(float wX, float wY, float wZ, float* o) {
     const float u = rng.next();
     const float v = rng.next();
     const float phi = DF_PI_2 * u;
@@ -710,7 +717,7 @@ struct Tracer {
   // ---------------------------------------------------------- renderers
 
   // Path tracer (:3151-3705).  Returns the sample colour in out[3].
-  __device__ void render_pt(const float ray[6], bool sky_active, bool sun_active, float out[3], int* prim0) {
+  __device__ __forceinline__ void render_pt(const float ray[6], bool sky_active, bool sun_active, float out[3], int* prim0) {
     const int depth_max = P.depth_max;
     const int depth_rr = P.depth_rr;
     int depth = 0;
@@ -718,7 +725,7 @@ struct Tracer {
     float pr = 0.0f, pg = 0.0f, pb = 0.0f;
     float mr = 1.0f, mg = 1.0f, mb = 1.0f;
     do {
-      intersect(ox, oy, oz, dx, dy, dz, false);
+      intersect_t<false>(ox, oy, oz, dx, dy, dz);
       const int po = h.prim;
       if (depth == 0) *prim0 = po;
       if (h.t == DF_INF || po == -1) {
@@ -757,7 +764,7 @@ struct Tracer {
         const float pdf = jmax(aR, jmax(aG, aB));
         const float rnd = rng.next();
         if (rnd >= pdf) {
-          const bool hit = intersect(ox, oy, oz, dx, dy, dz, true) < DF_INF;
+          const bool hit = intersect_t<true>(ox, oy, oz, dx, dy, dz) < DF_INF;
           if (!hit) {
             sky(sky_active, dx, dy, dz);
             pr += mr * tmp[0] * DF_PI_RECIP;
@@ -917,7 +924,7 @@ struct Tracer {
         mb *= aB;
       }
     } while (depth < depth_max);
-    const bool hit = intersect(ox, oy, oz, dx, dy, dz, true) < DF_INF;
+    const bool hit = intersect_t<true>(ox, oy, oz, dx, dy, dz) < DF_INF;
     if (!hit) {
       sky(sky_active, dx, dy, dz);
       pr += mr * tmp[0] * DF_PI_RECIP;
@@ -930,10 +937,10 @@ struct Tracer {
   }
 
   // Ambient occlusion (:3029-3149).
-  __device__ void render_ao(const float ray[6], float out[3], int* prim0) {
+  __device__ __forceinline__ void render_ao(const float ray[6], float out[3], int* prim0) {
     const float dx = ray[3], dy = ray[4], dz = ray[5];
     float pr = 0.0f, pg = 0.0f, pb = 0.0f;
-    intersect(ray[0], ray[1], ray[2], dx, dy, dz, false);
+    intersect_t<false>(ray[0], ray[1], ray[2], dx, dy, dz);
     const int po = h.prim;
     *prim0 = po;
     if (h.t != DF_INF && po != -1) {
@@ -963,7 +970,7 @@ struct Tracer {
       const float d1Y = d0X * u2Y + d0Y * v0Y + d0Z * w1Y;
       const float d1Z = d0X * u2Z + d0Y * v0Z + d0Z * w1Z;
       const float dr = jrsqrt(d1X * d1X + d1Y * d1Y + d1Z * d1Z);
-      const float t = intersect(pX + w1X * 0.01f, pY + w1Y * 0.01f, pZ + w1Z * 0.01f, d1X * dr, d1Y * dr, d1Z * dr, false);
+      const float t = intersect_t<false>(pX + w1X * 0.01f, pY + w1Y * 0.01f, pZ + w1Z * 0.01f, d1X * dr, d1Y * dr, d1Z * dr);
       const bool hit = t < P.ao_max;
       const float r = hit ? 1.0f : 0.0f;
       pr += (1.0f - r) * 1.0f + r * 0.0f;
@@ -976,9 +983,9 @@ struct Tracer {
   }
 
   // Ray caster (:3707-3800).
-  __device__ void render_rc(const float ray[6], bool sky_active, float out[3], int* prim0) {
+  __device__ __forceinline__ void render_rc(const float ray[6], bool sky_active, float out[3], int* prim0) {
     const float dx = ray[3], dy = ray[4], dz = ray[5];
-    intersect(ray[0], ray[1], ray[2], dx, dy, dz, false);
+    intersect_t<false>(ray[0], ray[1], ray[2], dx, dy, dz);
     const int po = h.prim;
     *prim0 = po;
     if (h.t == DF_INF || po == -1) {
@@ -1001,12 +1008,12 @@ struct Tracer {
   }
 
   // Ray tracer (:3866-3996).
-  __device__ void render_rt(const float ray[6], bool sky_active, float out[3], int* prim0) {
+  __device__ __forceinline__ void render_rt(const float ray[6], bool sky_active, float out[3], int* prim0) {
     int depth = 0;
     float ox = ray[0], oy = ray[1], oz = ray[2], dx = ray[3], dy = ray[4], dz = ray[5];
     float pr = 0.0f, pg = 0.0f, pb = 0.0f;
     do {
-      intersect(ox, oy, oz, dx, dy, dz, false);
+      intersect_t<false>(ox, oy, oz, dx, dy, dz);
       const int po = h.prim;
       if (depth == 0) *prim0 = po;
       if (h.t == DF_INF || po == -1) {
@@ -1050,9 +1057,9 @@ struct Tracer {
   }
 
   // Surface normals (:2848-2908).
-  __device__ void render_sn(const float ray[6], bool sky_active, float out[3], int* prim0) {
+  __device__ __forceinline__ void render_sn(const float ray[6], bool sky_active, float out[3], int* prim0) {
     const float dx = ray[3], dy = ray[4], dz = ray[5];
-    intersect(ray[0], ray[1], ray[2], dx, dy, dz, false);
+    intersect_t<false>(ray[0], ray[1], ray[2], dx, dy, dz);
     const int po = h.prim;
     *prim0 = po;
     if (h.t == DF_INF || po == -1) {
@@ -1068,9 +1075,9 @@ struct Tracer {
   }
 
   // Ray marcher over the global simplex fractal (:3802-3864).
-  __device__ float get_y(float x, float z) const { return noise.fractal2(P.ga, P.gf, P.gg, P.gl, P.go, x, z); }
+  __device__ __forceinline__ float get_y(float x, float z) const { return noise.fractal2(P.ga, P.gf, P.gg, P.gl, P.go, x, z); }
 
-  __device__ void render_rm(const float ray[6], bool sky_active, float out[3]) {
+  __device__ __forceinline__ void render_rm(const float ray[6], bool sky_active, float out[3]) {
     const float ox = ray[0], oz = ray[2];
     const float oy = get_y(ox, oz) + 0.1f;
     const float dx = ray[3], dy = ray[4], dz = ray[5];
@@ -1102,8 +1109,8 @@ struct Tracer {
   }
 
   // Wireframe overlay (:2910-3027); returns true when the pixel is on an edge.
-  __device__ bool wireframe_edge(const float ray[6]) {
-    intersect(ray[0], ray[1], ray[2], ray[3], ray[4], ray[5], false);
+  __device__ __forceinline__ bool wireframe_edge(const float ray[6]) {
+    intersect_t<false>(ray[0], ray[1], ray[2], ray[3], ray[4], ray[5]);
     const float lwc = Math<M>::cos(DF_PI * 0.5f / 4096.0f);
     if (!(h.t != DF_INF && h.prim != -1) || h.shape_type != 4) return false;
     const float* c = C.c;
