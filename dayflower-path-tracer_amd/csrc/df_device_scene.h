@@ -69,8 +69,11 @@ struct DScene {
   const DPrim* prims;
   const DNode* nodes;
   const DTri* tris;
+  // Terrain march positions tc_0 = 0.001, tc_{k+1} = tc_k + 0.01 * tc_k (float
+  // arithmetic, every tc_k < 10): the sequence every terrain query visits.
+  const float* march_tc;
   int32_t prim_count;
-  int32_t pad0;
+  int32_t march_n;
 };
 
 struct DCamera {

@@ -460,6 +460,19 @@ df_params reference_defaults() {
   return p;
 }
 
+// The positions of the terrain height march (GPURendererKernel.java:1484-1501:
+// tc starts at 0.001 and grows by td = 0.01 * tc while tc < 10), in the same
+// float arithmetic as the device loop; they do not depend on the ray.
+std::vector<float> terrain_march_positions() {
+  std::vector<float> v;
+  float td = 0.01f;
+  for (float tc = 0.001f; tc < 10.0f; tc += td) {
+    v.push_back(tc);
+    td = 0.01f * tc;
+  }
+  return v;
+}
+
 int rebuild_device_scene(df_ctx* c) {
   std::vector<DPrim> prims;
   std::vector<DNode> nodes;
@@ -468,7 +481,9 @@ int rebuild_device_scene(df_ctx* c) {
   if (rc) return rc;
   // buffers: 0 point2fs 1 point3fs 2 vector3fs 3 planes 4 spheres 5 terrains
   // 6 triangles 7 textures 8 surfaces 9 primitives 10 prims 11 nodes 12 tris
-  c->scene_bufs.resize(13);
+  // 13 terrain march positions
+  c->scene_bufs.resize(14);
+  const std::vector<float> march = terrain_march_positions();
   const HostScene& h = c->hs;
   if ((rc = upload(c, c->scene_bufs[0], h.point2fs)) || (rc = upload(c, c->scene_bufs[1], h.point3fs)) ||
       (rc = upload(c, c->scene_bufs[2], h.vector3fs)) || (rc = upload(c, c->scene_bufs[3], h.planes)) ||
@@ -476,7 +491,7 @@ int rebuild_device_scene(df_ctx* c) {
       (rc = upload(c, c->scene_bufs[6], h.triangles)) || (rc = upload(c, c->scene_bufs[7], h.textures)) ||
       (rc = upload(c, c->scene_bufs[8], h.surfaces)) || (rc = upload(c, c->scene_bufs[9], h.primitives)) ||
       (rc = upload(c, c->scene_bufs[10], prims)) || (rc = upload(c, c->scene_bufs[11], nodes)) ||
-      (rc = upload(c, c->scene_bufs[12], tris)))
+      (rc = upload(c, c->scene_bufs[12], tris)) || (rc = upload(c, c->scene_bufs[13], march)))
     return rc;
   DScene& d = c->ds;
   d.point2fs = static_cast<const float*>(c->scene_bufs[0].p);
@@ -492,7 +507,9 @@ int rebuild_device_scene(df_ctx* c) {
   d.prims = static_cast<const DPrim*>(c->scene_bufs[10].p);
   d.nodes = static_cast<const DNode*>(c->scene_bufs[11].p);
   d.tris = static_cast<const DTri*>(c->scene_bufs[12].p);
+  d.march_tc = static_cast<const float*>(c->scene_bufs[13].p);
   d.prim_count = static_cast<int32_t>(prims.size());
+  d.march_n = static_cast<int32_t>(march.size());
   return DF_OK;
 }
 

@@ -172,7 +172,7 @@ struct Tracer {
         ++sphere_tests;
         cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
       } else if (st == 3) {
-        cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, &terrain_steps);
+        cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n, &terrain_steps);
       } else if (st == 4) {
         const DTri tr = S.tris[pr.node_root];
         ++tri_tests;
