@@ -28,6 +28,11 @@ struct DPrim {
   float w2o[12];         // world -> object rows 1..3
   float o2w[12];         // object -> world rows 1..3
   float geo[8];          // sphere: pos.xyz, radius; plane: a.xyz, n.xyz; triangle: unused
+  // World-space box enclosing every point the primitive's test can return
+  // (enlarged to cover float rounding); bounded = 0 for planes and terrains.
+  float wb[6];           // min.xyz, max.xyz
+  int32_t bounded;
+  int32_t pad[3];
 };
 
 // Repacked BVH node (pre-order).  box = (min.xyz, max.xyz); next = index of
@@ -129,8 +134,37 @@ struct DFrame {
   uint8_t* image_bgra;     // 4 * W * H
   int32_t* primitive_offsets;
   uint64_t* seeds;
-  unsigned long long* counters;  // df_counters order: rays, samples, box, tri, prim, sphere, terrain steps, sky
+  unsigned long long* counters;  // per stage: rays, samples, box, tri, prim, sphere, terrain, sky
 
 };
+
+// Path state of the wavefront path tracer, one slot per pixel (y * W + x),
+// and its work queues.  float4 records keep each slot one 16-byte access.
+struct DPaths {
+  float4* ray_o;      // current ray origin
+  float4* ray_d;      // current ray direction
+  float4* thru;       // path throughput
+  float4* rad;        // radiance so far (without the sun term while its shadow query is pending)
+  float4* hit;        // closest hit: t, primitive index, shape type, shape offset (int bits)
+  float4* sh_o;       // shadow query origin
+  float4* sh_d;       // shadow query direction
+  float4* rad_lit;    // radiance with the sun term, taken when the shadow query hits nothing
+  float4* sample;     // the pass's sample colour; w = 1: set the film instead of accumulating
+  float4* ray0_o;     // primary ray, kept for the wireframe test
+  float4* ray0_d;
+  int32_t* queue[2];  // live paths of even / odd bounces
+  int32_t* shadow_queue;
+  int32_t* ctl;       // per-pass queue counts and work heads
+  int64_t n;          // slots
+  int32_t ctl_len;
+  int32_t pad;
+};
+
+// Length of DPaths::ctl for a maximum depth: B + 1 live-path counts, B
+// shadow counts and 3B + 3 work heads, B = max(1, depth).
+__host__ __device__ inline int wavefront_ctl_len(int depth_max) {
+  const int b = depth_max > 1 ? depth_max : 1;
+  return 5 * b + 4;
+}
 
 }  // namespace dfk

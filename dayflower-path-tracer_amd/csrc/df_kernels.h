@@ -1,11 +1,25 @@
-// Launch wrappers exported by df_kernels.hip to the host runtime.
+// Launch wrappers exported by df_kernels.hip and df_wavefront.hip to the host runtime.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "df_device_scene.h"
 
+// Optional hook around each kernel launch (stage = DF_STAGE_*): begin = 1 is
+// called just before the launch is enqueued, begin = 0 just after.
+struct dfk_marks {
+  void (*mark)(void* user, int stage, int begin);
+  void* user;
+};
+
 hipError_t dfk_launch_render(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k, const dfk::DParams& p,
-                             const dfk::DFrame& f, int math_mode, hipStream_t stream);
+                             const dfk::DFrame& f, int math_mode, hipStream_t stream,
+                             const dfk_marks* marks = nullptr);
 
 hipError_t dfk_launch_seed(uint64_t* seeds, uint64_t n, uint64_t master, hipStream_t stream);
+
+// One or more passes of the path tracer as wavefront kernels (df_wavefront.hip);
+// w must hold at least width * height slots and wavefront_ctl_len(depth) counters.
+hipError_t dfk_launch_pt_wavefront(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k,
+                                   const dfk::DParams& p, const dfk::DFrame& f, const dfk::DPaths& w, int math_mode,
+                                   hipStream_t stream, const dfk_marks* marks = nullptr);

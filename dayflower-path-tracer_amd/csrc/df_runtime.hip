@@ -6,6 +6,7 @@
 // launches the kernels on one HIP stream per handle.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -63,9 +64,29 @@ struct df_ctx {
   int width = 0, height = 0;
   DevBuf film, film_samples, image_float, image_bgra, prim_offsets, seeds, counters;
   bool clear_pending = false;
+
+  // wavefront path state (allocated on first use for the frame size)
+  DevBuf path_f4[11], path_q[3], path_ctl;
+  dfk::DPaths paths{};
+
+  // per-stage launch timing: event pairs recorded around each launch
+  bool stage_timing = false;
+  hipError_t mark_error = hipSuccess;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  struct Span {
+    int stage;
+    size_t begin, end;
+  };
+  std::vector<Span> spans;
+  size_t open_begin = 0;
+  double stage_ms[DF_STAGE_COUNT] = {};
+  int64_t stage_launches[DF_STAGE_COUNT] = {};
 };
 
 namespace {
+
+constexpr size_t kCounterBytes = sizeof(df_counters) * DF_STAGE_COUNT;
 
 int fail(df_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -277,6 +298,87 @@ int repack_mesh(df_ctx* c, const HostScene& h, int32_t base, std::vector<DNode>&
   return DF_OK;
 }
 
+// World-space bounds of a primitive, for the ray query's bounds cull.  The
+// kernel tests the world ray against them before the reference's per-shape
+// test: the ray parameter t is the same in world and object space (the
+// object ray is the world ray under the affine world->object map), so the
+// box's t-interval in world space contains the object-space interval of the
+// shape's box whenever the box maps onto the object box.  The object box is
+// mapped with the inverse of world_to_object computed here in double (the
+// kernel transforms rays with world_to_object), and enlarged so that the
+// float rounding of both tests stays inside it.
+void world_bounds(const HostScene& h, const std::vector<DNode>& nodes, DPrim& p) {
+  p.bounded = 0;
+  double lo[3], hi[3];
+  switch (p.type) {
+    case 2:  // sphere: centre +- radius
+      for (int k = 0; k < 3; ++k) {
+        const double r = std::fabs(static_cast<double>(p.geo[3]));
+        lo[k] = p.geo[k] - r;
+        hi[k] = p.geo[k] + r;
+      }
+      break;
+    case 4: {  // triangle: its vertices
+      const int32_t* t = h.triangles.data() + p.shape_offset;
+      for (int k = 0; k < 3; ++k) lo[k] = hi[k] = h.point3fs[t[0] + k];
+      for (int v = 1; v < 3; ++v)
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], static_cast<double>(h.point3fs[t[v] + k]));
+          hi[k] = std::max(hi[k], static_cast<double>(h.point3fs[t[v] + k]));
+        }
+      break;
+    }
+    case 5: {  // mesh: the root box (a ray missing it reaches no triangle)
+      const DNode& r = nodes[p.node_root];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = r.mn[k];
+        hi[k] = r.mx[k];
+      }
+      break;
+    }
+    default:
+      return;
+  }
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]) || lo[k] > hi[k]) return;
+  // inverse of the affine world->object map [A | b]: x = A^-1 (y - b)
+  const float* m = p.w2o;
+  const double a[3][3] = {{m[0], m[1], m[2]}, {m[4], m[5], m[6]}, {m[8], m[9], m[10]}};
+  const double b[3] = {m[3], m[7], m[11]};
+  const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                     a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                     a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+  if (!std::isfinite(det) || std::fabs(det) < 1e-12) return;
+  double inv[3][3];
+  inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+  inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+  inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+  inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+  inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+  inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+  inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+  inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+  inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+  double wlo[3] = {1e300, 1e300, 1e300}, whi[3] = {-1e300, -1e300, -1e300};
+  for (int corner = 0; corner < 8; ++corner) {
+    const double y[3] = {(corner & 1) ? hi[0] : lo[0], (corner & 2) ? hi[1] : lo[1], (corner & 4) ? hi[2] : lo[2]};
+    for (int r = 0; r < 3; ++r) {
+      const double x = inv[r][0] * (y[0] - b[0]) + inv[r][1] * (y[1] - b[1]) + inv[r][2] * (y[2] - b[2]);
+      wlo[r] = std::min(wlo[r], x);
+      whi[r] = std::max(whi[r], x);
+    }
+  }
+  double scale = 1.0;
+  for (int k = 0; k < 3; ++k) scale = std::max(scale, std::max(std::fabs(wlo[k]), std::fabs(whi[k])));
+  const double pad = 1e-4 * scale + 1e-3;
+  for (int k = 0; k < 3; ++k) {
+    if (!std::isfinite(wlo[k]) || !std::isfinite(whi[k])) return;
+    p.wb[k] = static_cast<float>(wlo[k] - pad);
+    p.wb[3 + k] = static_cast<float>(whi[k] + pad);
+  }
+  p.bounded = 1;
+}
+
 int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::vector<DNode>& nodes,
                 std::vector<DTri>& tris) {
   const size_t np = h.primitives.size() / 3;
@@ -336,6 +438,7 @@ int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::v
       default:
         break;  // unknown shapes are never hit (the reference skips them)
     }
+    world_bounds(h, nodes, p);
   }
   return DF_OK;
 }
@@ -436,6 +539,7 @@ int validate_params(df_ctx* c, const df_params& p) {
   if (p.pt_depth_maximum > 256) return fail(c, DF_EINVAL, "pt_depth_maximum %d > 256", p.pt_depth_maximum);
   if (p.global_octaves < 0 || p.global_octaves > 64) return fail(c, DF_EINVAL, "global_octaves %d", p.global_octaves);
   if (p.math_mode != DF_MATH_FAST && p.math_mode != DF_MATH_EXACT) return fail(c, DF_EINVAL, "math_mode %d", p.math_mode);
+  if (p.pt_engine < DF_PT_AUTO || p.pt_engine > DF_PT_WAVEFRONT) return fail(c, DF_EINVAL, "pt_engine %d", p.pt_engine);
   return DF_OK;
 }
 
@@ -457,6 +561,7 @@ df_params reference_defaults() {
   p.global_lacunarity = 2.0f;
   p.global_octaves = 2;
   p.math_mode = DF_MATH_FAST;
+  p.pt_engine = DF_PT_AUTO;
   return p;
 }
 
@@ -520,6 +625,88 @@ int need_frame(df_ctx* c) {
 }
 
 size_t pixels(const df_ctx* c) { return static_cast<size_t>(c->width) * static_cast<size_t>(c->height); }
+
+void free_paths(df_ctx* c) {
+  for (DevBuf& b : c->path_f4) free_buf(b);
+  for (DevBuf& b : c->path_q) free_buf(b);
+  free_buf(c->path_ctl);
+  c->paths = dfk::DPaths{};
+}
+
+// Path state of the wavefront path tracer for the current frame size and a
+// maximum depth (11 float4 records + 3 queue entries per pixel: 188 bytes).
+int ensure_paths(df_ctx* c, int depth_max) {
+  const size_t n = pixels(c);
+  const int ctl_len = dfk::wavefront_ctl_len(depth_max);
+  if (c->paths.n == static_cast<int64_t>(n) && c->paths.ctl_len >= ctl_len) return DF_OK;
+  free_paths(c);
+  int rc;
+  for (DevBuf& b : c->path_f4)
+    if ((rc = alloc_buf(c, b, n * sizeof(float4)))) return rc;
+  for (DevBuf& b : c->path_q)
+    if ((rc = alloc_buf(c, b, n * sizeof(int32_t)))) return rc;
+  const int ctl_alloc = ctl_len > 2048 ? ctl_len : 2048;
+  if ((rc = alloc_buf(c, c->path_ctl, static_cast<size_t>(ctl_alloc) * sizeof(int32_t)))) return rc;
+  dfk::DPaths& w = c->paths;
+  float4** f4[11] = {&w.ray_o, &w.ray_d, &w.thru, &w.rad, &w.hit, &w.sh_o, &w.sh_d, &w.rad_lit, &w.sample,
+                     &w.ray0_o, &w.ray0_d};
+  for (int i = 0; i < 11; ++i) *f4[i] = static_cast<float4*>(c->path_f4[i].p);
+  w.queue[0] = static_cast<int32_t*>(c->path_q[0].p);
+  w.queue[1] = static_cast<int32_t*>(c->path_q[1].p);
+  w.shadow_queue = static_cast<int32_t*>(c->path_q[2].p);
+  w.ctl = static_cast<int32_t*>(c->path_ctl.p);
+  w.n = static_cast<int64_t>(n);
+  w.ctl_len = ctl_alloc;
+  return DF_OK;
+}
+
+// The path tracer's schedule for this frame.
+bool use_wavefront(const df_ctx* c) {
+  if (c->params.renderer_type != DF_RENDERER_PATH_TRACER) return false;
+  if (c->params.pt_engine == DF_PT_WAVEFRONT) return true;
+  if (c->params.pt_engine == DF_PT_MEGAKERNEL) return false;
+  return pixels(c) >= (1u << 18);
+}
+
+hipEvent_t take_event(df_ctx* c) {
+  if (c->ev_used == c->ev_pool.size()) {
+    hipEvent_t e = nullptr;
+    const hipError_t r = hipEventCreate(&e);
+    if (r != hipSuccess) {
+      c->mark_error = r;
+      return nullptr;
+    }
+    c->ev_pool.push_back(e);
+  }
+  return c->ev_pool[c->ev_used++];
+}
+
+// dfk_marks hook: an event before and after each launch.
+void stage_mark(void* user, int stage, int begin) {
+  df_ctx* c = static_cast<df_ctx*>(user);
+  hipEvent_t e = take_event(c);
+  if (!e) return;
+  const hipError_t r = hipEventRecord(e, c->stream);
+  if (r != hipSuccess) c->mark_error = r;
+  if (begin) {
+    c->open_begin = c->ev_used - 1;
+  } else {
+    c->spans.push_back({stage, c->open_begin, c->ev_used - 1});
+  }
+}
+
+// Folds the recorded spans into the per-stage totals (after a stream sync).
+int collect_spans(df_ctx* c) {
+  for (const df_ctx::Span& sp : c->spans) {
+    float ms = 0.0f;
+    DF_HIP(c, hipEventElapsedTime(&ms, c->ev_pool[sp.begin], c->ev_pool[sp.end]));
+    c->stage_ms[sp.stage] += ms;
+    c->stage_launches[sp.stage] += 1;
+  }
+  c->spans.clear();
+  c->ev_used = 0;
+  return DF_OK;
+}
 
 int read_back(df_ctx* c, void* dst, const DevBuf& b, size_t bytes) {
   DF_HIP(c, hipStreamSynchronize(c->stream));
@@ -588,8 +775,7 @@ int df_create(df_ctx** out, int32_t device) {
     df_destroy(c);
     return DF_EHIP;
   }
-  if (alloc_buf(c, c->counters, sizeof(df_counters)) != DF_OK ||
-      hipMemset(c->counters.p, 0, sizeof(df_counters)) != hipSuccess) {
+  if (alloc_buf(c, c->counters, kCounterBytes) != DF_OK || hipMemset(c->counters.p, 0, kCounterBytes) != hipSuccess) {
     df_destroy(c);
     return DF_ENOMEM;
   }
@@ -609,6 +795,8 @@ void df_destroy(df_ctx* c) {
   free_buf(c->prim_offsets);
   free_buf(c->seeds);
   free_buf(c->counters);
+  free_paths(c);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -654,6 +842,7 @@ int df_update(df_ctx* c, int32_t w, int32_t h, int32_t local_size_hint) {
   DF_HIP(c, hipSetDevice(c->device));
   DF_HIP(c, hipStreamSynchronize(c->stream));
   const size_t n = static_cast<size_t>(w) * h;
+  free_paths(c);
   int rc;
   if ((rc = alloc_buf(c, c->film, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->film_samples, n * sizeof(int32_t))) ||
       (rc = alloc_buf(c, c->image_float, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->image_bgra, n * 4)) ||
@@ -815,9 +1004,19 @@ int df_execute(df_ctx* c, int32_t passes) {
   f.primitive_offsets = static_cast<int32_t*>(c->prim_offsets.p);
   f.seeds = static_cast<uint64_t*>(c->seeds.p);
   f.counters = static_cast<unsigned long long*>(c->counters.p);
+  const bool wavefront = use_wavefront(c);
+  if (wavefront && (rc = ensure_paths(c, c->params.pt_depth_maximum))) return rc;
+  const dfk_marks marks{stage_mark, c};
+  const dfk_marks* mk = c->stage_timing ? &marks : nullptr;
+  c->mark_error = hipSuccess;
   DF_HIP(c, hipEventRecord(c->ev0, c->stream));
-  DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, c->params.math_mode, c->stream));
+  if (wavefront) {
+    DF_HIP(c, dfk_launch_pt_wavefront(c->ds, cam, sky, par, f, c->paths, c->params.math_mode, c->stream, mk));
+  } else {
+    DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, c->params.math_mode, c->stream, mk));
+  }
   DF_HIP(c, hipEventRecord(c->ev1, c->stream));
+  if (c->mark_error != hipSuccess) return hip_fail(c, c->mark_error, "stage timing event");
   c->timed = true;
   c->clear_pending = false;  // clearFilmFlags() after the pass
   return DF_OK;
@@ -883,14 +1082,50 @@ int df_set_film(df_ctx* c, const float* rgb, size_t rgb_n, const int32_t* sample
 int df_get_counters(df_ctx* c, df_counters* out) {
   if (!c || !out) return DF_EINVAL;
   DF_HIP(c, hipStreamSynchronize(c->stream));
-  DF_HIP(c, hipMemcpy(out, c->counters.p, sizeof(df_counters), hipMemcpyDeviceToHost));
+  uint64_t all[DF_STAGE_COUNT][8];
+  DF_HIP(c, hipMemcpy(all, c->counters.p, kCounterBytes, hipMemcpyDeviceToHost));
+  uint64_t sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int st = 0; st < DF_STAGE_COUNT; ++st)
+    for (int i = 0; i < 8; ++i) sum[i] += all[st][i];
+  std::memcpy(out, sum, sizeof sum);
   return DF_OK;
 }
 
 int df_reset_counters(df_ctx* c) {
   if (!c) return DF_EINVAL;
   DF_HIP(c, hipStreamSynchronize(c->stream));
-  DF_HIP(c, hipMemset(c->counters.p, 0, sizeof(df_counters)));
+  DF_HIP(c, hipMemset(c->counters.p, 0, kCounterBytes));
+  c->spans.clear();
+  c->ev_used = 0;
+  for (int st = 0; st < DF_STAGE_COUNT; ++st) {
+    c->stage_ms[st] = 0.0;
+    c->stage_launches[st] = 0;
+  }
+  return DF_OK;
+}
+
+int df_set_stage_timing(df_ctx* c, int32_t enabled) {
+  if (!c) return DF_EINVAL;
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = collect_spans(c);
+  if (rc) return rc;
+  c->stage_timing = enabled != 0;
+  return DF_OK;
+}
+
+int df_get_stage_stats(df_ctx* c, int32_t stage, df_counters* counters, double* ms, int64_t* launches) {
+  if (!c) return DF_EINVAL;
+  if (stage < 0 || stage >= DF_STAGE_COUNT) return fail(c, DF_EINVAL, "stage %d", stage);
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  int rc = collect_spans(c);
+  if (rc) return rc;
+  if (counters) {
+    uint64_t all[DF_STAGE_COUNT][8];
+    DF_HIP(c, hipMemcpy(all, c->counters.p, kCounterBytes, hipMemcpyDeviceToHost));
+    std::memcpy(counters, all[stage], sizeof(df_counters));
+  }
+  if (ms) *ms = c->stage_ms[stage];
+  if (launches) *launches = c->stage_launches[stage];
   return DF_OK;
 }
 

@@ -112,9 +112,22 @@ struct Tracer {
 
   template <bool testing>
   __device__ __forceinline__ float intersect_t(float ox, float oy, float oz, float dx, float dy, float dz) {
+    int ci, cst, cso;
+    const float closest = query<testing>(ox, oy, oz, dx, dy, dz, ci, cst, cso);
+    if (!testing) fill_hit(ox, oy, oz, dx, dy, dz, closest, ci, cst, cso);
+    return closest;
+  }
+
+  // The primitive loop of doIntersectPrimitives: the closest (or, testing,
+  // the first) hit's distance, primitive index, shape type and shape offset.
+  template <bool testing>
+  __device__ __forceinline__ float query(float ox, float oy, float oz, float dx, float dy, float dz, int& ci,
+                                         int& cst, int& cso) {
     ++rays;
     float closest = DF_INF;
-    int ci = -1, cst = -1, cso = -1;
+    ci = -1;
+    cst = -1;
+    cso = -1;
     const int n = S.prim_count;
     for (int i = 0; i < n; ++i) {
       const DPrim& pr = S.prims[i];
@@ -186,7 +199,14 @@ struct Tracer {
       }
       if (testing && ci != -1) return closest;
     }
-    if (!testing) {
+    return closest;
+  }
+
+  // The hit record of the query's result (the tail of doIntersectPrimitives):
+  // surface point, normals, texture coordinates, normal mapping.
+  __device__ __forceinline__ void fill_hit(float ox, float oy, float oz, float dx, float dy, float dz,
+                                           float closest, int ci, int cst, int cso) {
+    {
       if (ci != -1) {
         const DPrim& pr = S.prims[ci];
         const float* m = pr.w2o;
@@ -219,7 +239,6 @@ struct Tracer {
         h.shape_off = -1;
       }
     }
-    return closest;
   }
 
   // ---------------------------------------------------------- surface properties
@@ -644,8 +663,12 @@ struct Tracer {
     tmp[2] = b;
   }
 
-  __device__ __forceinline__ void sun(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ, float aR, float aG,
-                      float aB) {  // :2238-2347
+  // Samples a direction towards the sun.  Returns true when a shadow query
+  // towards `sd` decides the result; tmp then holds the contribution when
+  // nothing is hit (and 0 otherwise, as it also does on false).
+  __device__ __forceinline__ bool sun_sample(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ,
+                                             float aR, float aG, float aB, float sd[3]) {  // :2238-2347
+    bool query_needed = false;
     float r = 0.0f, g = 0.0f, b = 0.0f;
     if (active) {
       const float sdX = K.sun_dir_world[0], sdY = K.sun_dir_world[1], sdZ = K.sun_dir_world[2];
@@ -678,17 +701,24 @@ struct Tracer {
       const float dp0 = dX * sdX + dY * sdY + dZ * sdZ;
       const float dp1 = dX * nX + dY * nY + dZ * nZ;
       if (dp0 > 0.0f && dp1 > 0.0f) {
-        const float t = intersect_t<true>(pX, pY, pZ, dX, dY, dZ);
-        if (t >= DF_INF - 0.0001f) {
-          r = aR * 1.0f * dp1 * DF_PI_RECIP;
-          g = aG * 1.0f * dp1 * DF_PI_RECIP;
-          b = aB * 1.0f * dp1 * DF_PI_RECIP;
-        }
+        query_needed = true;
+        sd[0] = dX;
+        sd[1] = dY;
+        sd[2] = dZ;
+        r = aR * 1.0f * dp1 * DF_PI_RECIP;
+        g = aG * 1.0f * dp1 * DF_PI_RECIP;
+        b = aB * 1.0f * dp1 * DF_PI_RECIP;
       }
     }
     tmp[0] = r;
     tmp[1] = g;
     tmp[2] = b;
+    return query_needed;
+  }
+
+  // The shadow query's verdict: nothing hit (:2331).
+  __device__ __forceinline__ bool sun_visible(float pX, float pY, float pZ, const float sd[3]) {
+    return intersect_t<true>(pX, pY, pZ, sd[0], sd[1], sd[2]) >= DF_INF - 0.0001f;
   }
 
   // Cosine-weighted direction about w (:3410-3444, :3466-3500).
@@ -716,71 +746,88 @@ struct Tracer {
 
   // ---------------------------------------------------------- renderers
 
-  // Path tracer (:3151-3705).  Returns the sample colour in out[3].
-  __device__ __forceinline__ void render_pt(const float ray[6], bool sky_active, bool sun_active, float out[3], int* prim0) {
-    const int depth_max = P.depth_max;
+  // One bounce of the path tracer (:3196-3688), after the closest-hit query
+  // of the ray (o, d) has filled the hit record: sky on a miss, emission and
+  // sun light, Russian roulette, then the material's scattered ray.
+  //
+  // The sun's shadow query is handed back instead of traced: with `shadow`
+  // set the caller traces (h.p, sd) with the any-hit query and, when nothing
+  // is hit, replaces p by pv, the radiance including the sun's contribution
+  // (the two differ only in that one term, so either order of evaluation
+  // gives the reference's bits).  The Russian-roulette exit re-tests the
+  // path's current ray with the any-hit query (:3224-3231); that ray has just
+  // hit a surface, the any-hit query runs the closest-hit query's tests up to
+  // that surface's primitive, so it always hits and its sky term is never
+  // added: the query is counted, not traced.
+  //
+  // Returns true while the path continues with the new ray in (o, d) and
+  // throughput m; false when it has ended with radiance p (pending the shadow
+  // query).  depth counts the bounces taken.
+  __device__ __forceinline__ bool pt_bounce(float o[3], float d[3], float m[3], float p[3], int& depth,
+                                            bool sky_active, bool sun_active, bool& shadow, float sd[3],
+                                            float pv[3]) {
     const int depth_rr = P.depth_rr;
-    int depth = 0;
-    float ox = ray[0], oy = ray[1], oz = ray[2], dx = ray[3], dy = ray[4], dz = ray[5];
-    float pr = 0.0f, pg = 0.0f, pb = 0.0f;
-    float mr = 1.0f, mg = 1.0f, mb = 1.0f;
-    do {
-      intersect_t<false>(ox, oy, oz, dx, dy, dz);
-      const int po = h.prim;
-      if (depth == 0) *prim0 = po;
-      if (h.t == DF_INF || po == -1) {
-        sky(sky_active, dx, dy, dz);
-        pr += mr * tmp[0] * DF_PI_RECIP;
-        pg += mg * tmp[1] * DF_PI_RECIP;
-        pb += mb * tmp[2] * DF_PI_RECIP;
-        out[0] = pr;
-        out[1] = pg;
-        out[2] = pb;
-        return;
-      }
-      const DPrim& prm = S.prims[po / 3];
-      const int so = prm.surface_offset;
-      const float pX = h.p[0], pY = h.p[1], pZ = h.p[2];
-      const float sX = h.ns[0], sY = h.ns[1], sZ = h.ns[2];
-      const float dp = sX * dx + sY * dy + sZ * dz;
-      const float dp2 = dp * 2.0f;
-      const bool ok = dp < 0.0f;
-      const float wX = ok ? sX : -sX, wY = ok ? sY : -sY, wZ = ok ? sZ : -sZ;
-      const int alb = tex_color(jf2i(S.surfaces[so + 1]));
-      float aR = ch_r(alb), aG = ch_g(alb), aB = ch_b(alb);
-      const int emi = tex_color(jf2i(S.surfaces[so + 2]));
-      const float eR = ch_r(emi), eG = ch_g(emi), eB = ch_b(emi);
-      const int mat = jf2i(S.surfaces[so]);
-      if (mat == 1 || mat == 3 || mat == 4) {
-        sun(sun_active, pX, pY, pZ, wX, wY, wZ, aR, aG, aB);
-      } else {
-        tmp[0] = tmp[1] = tmp[2] = 0.0f;
-      }
+    float ox = o[0], oy = o[1], oz = o[2], dx = d[0], dy = d[1], dz = d[2];
+    float pr = p[0], pg = p[1], pb = p[2];
+    float mr = m[0], mg = m[1], mb = m[2];
+    shadow = false;
+    const int po = h.prim;
+    if (h.t == DF_INF || po == -1) {
+      sky(sky_active, dx, dy, dz);
+      pr += mr * tmp[0] * DF_PI_RECIP;
+      pg += mg * tmp[1] * DF_PI_RECIP;
+      pb += mb * tmp[2] * DF_PI_RECIP;
+      p[0] = pr;
+      p[1] = pg;
+      p[2] = pb;
+      return false;
+    }
+    const DPrim& prm = S.prims[po / 3];
+    const int so = prm.surface_offset;
+    const float pX = h.p[0], pY = h.p[1], pZ = h.p[2];
+    const float sX = h.ns[0], sY = h.ns[1], sZ = h.ns[2];
+    const float dp = sX * dx + sY * dy + sZ * dz;
+    const float dp2 = dp * 2.0f;
+    const bool ok = dp < 0.0f;
+    const float wX = ok ? sX : -sX, wY = ok ? sY : -sY, wZ = ok ? sZ : -sZ;
+    const int alb = tex_color(jf2i(S.surfaces[so + 1]));
+    float aR = ch_r(alb), aG = ch_g(alb), aB = ch_b(alb);
+    const int emi = tex_color(jf2i(S.surfaces[so + 2]));
+    const float eR = ch_r(emi), eG = ch_g(emi), eB = ch_b(emi);
+    const int mat = jf2i(S.surfaces[so]);
+    if (mat == 1 || mat == 3 || mat == 4) {
+      shadow = sun_sample(sun_active, pX, pY, pZ, wX, wY, wZ, aR, aG, aB, sd);
+    } else {
+      tmp[0] = tmp[1] = tmp[2] = 0.0f;
+    }
+    if (shadow) {
+      pv[0] = pr + mr * (eR + tmp[0]);
+      pv[1] = pg + mg * (eG + tmp[1]);
+      pv[2] = pb + mb * (eB + tmp[2]);
+      pr += mr * (eR + 0.0f);
+      pg += mg * (eG + 0.0f);
+      pb += mb * (eB + 0.0f);
+    } else {
       pr += mr * (eR + tmp[0]);
       pg += mg * (eG + tmp[1]);
       pb += mb * (eB + tmp[2]);
-      depth++;
-      if (depth >= depth_rr) {
-        const float pdf = jmax(aR, jmax(aG, aB));
-        const float rnd = rng.next();
-        if (rnd >= pdf) {
-          const bool hit = intersect_t<true>(ox, oy, oz, dx, dy, dz) < DF_INF;
-          if (!hit) {
-            sky(sky_active, dx, dy, dz);
-            pr += mr * tmp[0] * DF_PI_RECIP;
-            pg += mg * tmp[1] * DF_PI_RECIP;
-            pb += mb * tmp[2] * DF_PI_RECIP;
-          }
-          out[0] = pr;
-          out[1] = pg;
-          out[2] = pb;
-          return;
-        }
-        const float pdfr = 1.0f / pdf;
-        aR *= pdfr;
-        aG *= pdfr;
-        aB *= pdfr;
+    }
+    p[0] = pr;
+    p[1] = pg;
+    p[2] = pb;
+    depth++;
+    if (depth >= depth_rr) {
+      const float pdf = jmax(aR, jmax(aG, aB));
+      const float rnd = rng.next();
+      if (rnd >= pdf) {
+        ++rays;  // the any-hit re-test of (o, d): always a hit
+        return false;
       }
+      const float pdfr = 1.0f / pdf;
+      aR *= pdfr;
+      aG *= pdfr;
+      aB *= pdfr;
+    }
       if (mat == 1) {  // ClearCoat :3323-3464
         const float a = 1.5f - 1.0f, b = 1.5f + 1.0f, c = 1.0f / 1.5f;
         const float e = (a * a) / (b * b);
@@ -923,17 +970,60 @@ struct Tracer {
         mg *= aG;
         mb *= aB;
       }
-    } while (depth < depth_max);
-    const bool hit = intersect_t<true>(ox, oy, oz, dx, dy, dz) < DF_INF;
-    if (!hit) {
-      sky(sky_active, dx, dy, dz);
-      pr += mr * tmp[0] * DF_PI_RECIP;
-      pg += mg * tmp[1] * DF_PI_RECIP;
-      pb += mb * tmp[2] * DF_PI_RECIP;
+    o[0] = ox;
+    o[1] = oy;
+    o[2] = oz;
+    d[0] = dx;
+    d[1] = dy;
+    d[2] = dz;
+    m[0] = mr;
+    m[1] = mg;
+    m[2] = mb;
+    return true;
+  }
+
+  // Path tracer (:3151-3705).  Returns the sample colour in out[3].
+  __device__ __forceinline__ void render_pt(const float ray[6], bool sky_active, bool sun_active, float out[3],
+                                            int* prim0) {
+    const int depth_max = P.depth_max;
+    int depth = 0;
+    float o[3] = {ray[0], ray[1], ray[2]};
+    float d[3] = {ray[3], ray[4], ray[5]};
+    float m[3] = {1.0f, 1.0f, 1.0f};
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    for (;;) {
+      intersect_t<false>(o[0], o[1], o[2], d[0], d[1], d[2]);
+      if (depth == 0) *prim0 = h.prim;
+      bool shadow;
+      float sd[3], pv[3];
+      const bool more = pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv);
+      if (shadow && sun_visible(h.p[0], h.p[1], h.p[2], sd)) {
+        p[0] = pv[0];
+        p[1] = pv[1];
+        p[2] = pv[2];
+      }
+      if (!more) break;
+      if (depth >= depth_max) {
+        pt_escape(o, d, m, p, sky_active);
+        break;
+      }
     }
-    out[0] = pr;
-    out[1] = pg;
-    out[2] = pb;
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = p[2];
+  }
+
+  // The end of a path that took the maximum number of bounces: the sky's
+  // contribution if its last ray escapes (:3689-3700).
+  __device__ __forceinline__ void pt_escape(const float o[3], const float d[3], const float m[3], float p[3],
+                                            bool sky_active) {
+    const bool hit = intersect_t<true>(o[0], o[1], o[2], d[0], d[1], d[2]) < DF_INF;
+    if (!hit) {
+      sky(sky_active, d[0], d[1], d[2]);
+      p[0] += m[0] * tmp[0] * DF_PI_RECIP;
+      p[1] += m[1] * tmp[1] * DF_PI_RECIP;
+      p[2] += m[2] * tmp[2] * DF_PI_RECIP;
+    }
   }
 
   // Ambient occlusion (:3029-3149).

@@ -1,0 +1,337 @@
+// Wavefront path tracer for gfx950.
+//
+// The megakernel runs a whole path per work-item, so a wave holds lanes in
+// every stage at once (BVH walk, terrain march, texture noise, sky) and the
+// kernel's register footprint is that of the largest stage.  Here one pass
+// of GPURendererKernel.run() (:3151-3705 for the path tracer) is split into
+// kernels per stage, each over a queue of the paths still alive:
+//
+//   raygen           primary rays (camera, lens) for every pixel
+//   per bounce b:    shadow(b-1)  any-hit queries of the previous bounce's
+//                                 sun samples, resolving its radiance
+//                    closest(b)   closest-hit queries of the live paths
+//                    shade(b)     hit record, emission, sun sample, Russian
+//                                 roulette, scattered ray; appends the path
+//                                 to the next bounce's queue and its sun
+//                                 sample to the shadow queue
+//   shadow(last), escape          the sky test of paths that reached the
+//                                 maximum depth
+//   film             film accumulation, wireframes and the image pass
+//
+// Every path draws from its pixel's LCG stream in the reference's order
+// (raygen, then each shade), and the sun term is resolved from the two
+// radiance values computed by shade, so a pass gives the megakernel's bits.
+// The trace kernels are persistent: each wave takes 64 queue entries at a
+// time from an atomic head until the queue is drained, which balances the
+// very uneven cost of incoherent rays.  Queues are appended with one atomic
+// per wave.  Path state is indexed by pixel and lives in DPaths.
+#include <hip/hip_runtime.h>
+
+#include "df_kernels.h"
+#include "df_pass_common.h"
+
+namespace dfk {
+
+constexpr int32_t kFinish = 1 << 30;  // shadow-queue flag: the path ends once resolved
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ void stage_perm(int* perm) {
+  for (int i = threadIdx.x + threadIdx.y * blockDim.x; i < 512; i += blockDim.x * blockDim.y) perm[i] = kPerm[i & 255];
+  __syncthreads();
+}
+
+// Appends `value` for the lanes with `pred` set: one atomic per wave, the
+// lanes' entries in lane order.  Called by every active lane of the wave.
+__device__ __forceinline__ void wave_append(int32_t* q, int32_t* count, bool pred, int32_t value) {
+  const unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (pred) q[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// Runs body(valid, entry) over the queue, 64 entries per wave at a time.
+// Every lane of a wave takes part in every call (valid is false past the end).
+template <class Body>
+__device__ __forceinline__ void drain(const int32_t* q, const int32_t* count, int32_t* head, Body&& body) {
+  const int n = *count;
+  for (;;) {
+    int base = 0;
+    if (__lane_id() == 0) base = atomicAdd(head, 64);
+    base = __shfl(base, 0, 64);
+    if (base >= n) break;
+    const int k = base + __lane_id();
+    const bool valid = k < n;
+    body(valid, valid ? q[k] : 0);
+  }
+}
+
+__device__ __forceinline__ float4 f4(const float v[3], float w) { return make_float4(v[0], v[1], v[2], w); }
+
+__global__ void wf_reset(int32_t* ctl, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ctl[i] = 0;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W) {
+  const int x = blockIdx.x * 16 + threadIdx.x;
+  const int y = blockIdx.y * 16 + threadIdx.y;
+  bool valid = false;
+  int gid = 0;
+  if (x < P.width && y < P.height) {
+    gid = y * P.width + x;
+    Tracer<M> tr(S, C, K, P, kPerm);
+    tr.rng.s = F.seeds[gid];
+    float ray[6];
+    valid = tr.primary_ray(x, y, ray);
+    F.seeds[gid] = tr.rng.s;
+    if (valid) {
+      W.ray_o[gid] = make_float4(ray[0], ray[1], ray[2], 0.0f);
+      W.ray_d[gid] = make_float4(ray[3], ray[4], ray[5], 0.0f);
+      W.thru[gid] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+      W.rad[gid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (P.wireframes == 1) {
+        W.ray0_o[gid] = W.ray_o[gid];
+        W.ray0_d[gid] = W.ray_d[gid];
+      }
+    } else {
+      W.sample[gid] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // filmSetColor(0)
+    }
+  }
+  wave_append(W.queue[0], W.ctl, valid, gid);
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void wf_closest(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+                                                       const int32_t* q, const int32_t* count, int32_t* head,
+                                                       int first_bounce) {
+  Tracer<M> tr(S, C, K, P, kPerm);
+  drain(q, count, head, [&](bool valid, int32_t i) {
+    if (!valid) return;
+    const float4 o = W.ray_o[i], d = W.ray_d[i];
+    int ci, cst, cso;
+    const float t = tr.template query<false>(o.x, o.y, o.z, d.x, d.y, d.z, ci, cst, cso);
+    W.hit[i] = make_float4(t, __int_as_float(ci), __int_as_float(cst), __int_as_float(cso));
+    if (first_bounce) F.primitive_offsets[i] = ci == -1 ? -1 : ci * 3;
+  });
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+  take_counters(tr, cnt);
+  flush_counters(cnt, F.counters + kStageClosest * kCounterCount);
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+                                                     const int32_t* q, const int32_t* count, int32_t* head,
+                                                     int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
+                                                     int bounce) {
+  __shared__ int perm[512];
+  stage_perm(perm);
+  Tracer<M> tr(S, C, K, P, perm);
+  const bool sky_active = K.sky_active == 1;
+  const bool sun_active = K.sun_active == 1;
+  drain(q, count, head, [&](bool valid, int32_t i) {
+    bool more = false, shadow = false;
+    if (valid) {
+      const float4 o4 = W.ray_o[i], d4 = W.ray_d[i], hv = W.hit[i], m4 = W.thru[i], p4 = W.rad[i];
+      float o[3] = {o4.x, o4.y, o4.z}, d[3] = {d4.x, d4.y, d4.z};
+      float m[3] = {m4.x, m4.y, m4.z}, p[3] = {p4.x, p4.y, p4.z};
+      tr.fill_hit(o[0], o[1], o[2], d[0], d[1], d[2], hv.x, __float_as_int(hv.y), __float_as_int(hv.z),
+                  __float_as_int(hv.w));
+      tr.rng.s = F.seeds[i];
+      int depth = bounce;
+      float sd[3], pv[3];
+      more = tr.pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv);
+      F.seeds[i] = tr.rng.s;
+      W.rad[i] = f4(p, 0.0f);
+      if (shadow) {
+        W.sh_o[i] = f4(tr.h.p, 0.0f);
+        W.sh_d[i] = f4(sd, 0.0f);
+        W.rad_lit[i] = f4(pv, 0.0f);
+      }
+      if (more) {
+        W.ray_o[i] = f4(o, 0.0f);
+        W.ray_d[i] = f4(d, 0.0f);
+        W.thru[i] = f4(m, 0.0f);
+      } else if (!shadow) {
+        W.sample[i] = f4(p, 0.0f);
+      }
+    }
+    wave_append(q_next, count_next, valid && more, i);
+    wave_append(W.shadow_queue, count_shadow, valid && shadow, more ? i : (i | kFinish));
+  });
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+  take_counters(tr, cnt);
+  flush_counters(cnt, F.counters + kStageShade * kCounterCount);
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void wf_shadow(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+                                                      const int32_t* count, int32_t* head) {
+  Tracer<M> tr(S, C, K, P, kPerm);
+  drain(W.shadow_queue, count, head, [&](bool valid, int32_t e) {
+    if (!valid) return;
+    const int32_t i = e & (kFinish - 1);
+    const float4 o = W.sh_o[i], d = W.sh_d[i];
+    const float sd[3] = {d.x, d.y, d.z};
+    const bool lit = tr.sun_visible(o.x, o.y, o.z, sd);
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (lit) {
+      p = W.rad_lit[i];
+      W.rad[i] = p;
+    }
+    if (e & kFinish) {
+      if (!lit) p = W.rad[i];
+      W.sample[i] = make_float4(p.x, p.y, p.z, 0.0f);
+    }
+  });
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+  take_counters(tr, cnt);
+  flush_counters(cnt, F.counters + kStageShadow * kCounterCount);
+}
+
+template <int M>
+__global__ __launch_bounds__(kThreads) void wf_escape(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+                                                      const int32_t* q, const int32_t* count, int32_t* head) {
+  Tracer<M> tr(S, C, K, P, kPerm);
+  const bool sky_active = K.sky_active == 1;
+  drain(q, count, head, [&](bool valid, int32_t i) {
+    if (!valid) return;
+    const float4 o4 = W.ray_o[i], d4 = W.ray_d[i], m4 = W.thru[i], p4 = W.rad[i];
+    const float o[3] = {o4.x, o4.y, o4.z}, d[3] = {d4.x, d4.y, d4.z}, m[3] = {m4.x, m4.y, m4.z};
+    float p[3] = {p4.x, p4.y, p4.z};
+    tr.pt_escape(o, d, m, p, sky_active);
+    W.sample[i] = f4(p, 0.0f);
+  });
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+  take_counters(tr, cnt);
+  flush_counters(cnt, F.counters + kStageEscape * kCounterCount);
+}
+
+// filmAddColor / filmSetColor, the wireframe overlay and, after the last pass,
+// the image pass (run() :263-307).
+template <int M>
+__global__ __launch_bounds__(256) void wf_film(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W, int clear,
+                                               int last) {
+  __shared__ int perm[512];
+  if (P.wireframes == 1) stage_perm(perm);
+  const int x = blockIdx.x * 16 + threadIdx.x;
+  const int y = blockIdx.y * 16 + threadIdx.y;
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (x < P.width && y < P.height) {
+    const int gid = y * P.width + x;
+    Tracer<M> tr(S, C, K, P, perm);
+    const float4 col = W.sample[gid];
+    const bool set = col.w != 0.0f;
+    float film[3] = {F.film[gid * 3 + 0], F.film[gid * 3 + 1], F.film[gid * 3 + 2]};
+    int nsamp = F.film_samples[gid];
+    ++tr.samples;
+    if (set || clear) {
+      film[0] = col.x;
+      film[1] = col.y;
+      film[2] = col.z;
+      nsamp = 1;
+    } else {
+      const int n = nsamp + 1;
+      film[0] = film[0] + ((col.x - film[0]) / static_cast<float>(n));
+      film[1] = film[1] + ((col.y - film[1]) / static_cast<float>(n));
+      film[2] = film[2] + ((col.z - film[2]) / static_cast<float>(n));
+      nsamp = n;
+    }
+    if (!set && P.wireframes == 1) {
+      const float4 o = W.ray0_o[gid], d = W.ray0_d[gid];
+      const float ray[6] = {o.x, o.y, o.z, d.x, d.y, d.z};
+      if (tr.wireframe_edge(ray)) {
+        ++tr.samples;
+        film[0] = film[1] = film[2] = 0.0f;
+        nsamp = 1;
+      }
+    }
+    F.film[gid * 3 + 0] = film[0];
+    F.film[gid * 3 + 1] = film[1];
+    F.film[gid * 3 + 2] = film[2];
+    F.film_samples[gid] = nsamp;
+    if (last) image_pass<M>(P, film, F.primitive_offsets[gid], F.image_float + gid * 3, F.image_bgra + gid * 4);
+    take_counters(tr, cnt);
+  }
+  flush_counters(cnt, F.counters + kStageFilm * kCounterCount);
+}
+
+// Resident workgroups of a persistent kernel: occupancy x compute units.
+template <class Kernel>
+int resident_blocks(Kernel k) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kThreads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  return cus * per_cu;
+}
+
+template <int M>
+hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
+                            const DPaths& w, hipStream_t stream, const dfk_marks* marks) {
+  static const int g_closest = resident_blocks(wf_closest<M>);
+  static const int g_shade = resident_blocks(wf_shade<M>);
+  static const int g_shadow = resident_blocks(wf_shadow<M>);
+  static const int g_escape = resident_blocks(wf_escape<M>);
+  const int bounces = p.depth_max > 1 ? p.depth_max : 1;
+  const int ctl_len = wavefront_ctl_len(p.depth_max);
+  if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
+  // ctl: [0, B] live-path counts per bounce (B = escape queue), [B+1, 2B] shadow
+  // counts, then one work head per trace launch.
+  int32_t* ctl = w.ctl;
+  int32_t* shadow_count = ctl + bounces + 1;
+  int32_t* heads = ctl + 2 * bounces + 1;
+  const dim3 pix_block(16, 16);
+  const dim3 pix_grid((p.width + 15) / 16, (p.height + 15) / 16);
+  auto mark = [&](int stage, int begin) {
+    if (marks) marks->mark(marks->user, stage, begin);
+  };
+  for (int pass = 0; pass < p.passes; ++pass) {
+    hipLaunchKernelGGL(wf_reset, dim3((ctl_len + 255) / 256), dim3(256), 0, stream, ctl, ctl_len);
+    mark(kStageRaygen, 1);
+    hipLaunchKernelGGL(wf_raygen<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w);
+    mark(kStageRaygen, 0);
+    for (int b = 0; b < bounces; ++b) {
+      if (b > 0) {
+        mark(kStageShadow, 1);
+        hipLaunchKernelGGL(wf_shadow<M>, dim3(g_shadow), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                           shadow_count + (b - 1), heads + 3 * b + 2);
+        mark(kStageShadow, 0);
+      }
+      mark(kStageClosest, 1);
+      hipLaunchKernelGGL(wf_closest<M>, dim3(g_closest), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
+                         ctl + b, heads + 3 * b, b == 0 ? 1 : 0);
+      mark(kStageClosest, 0);
+      mark(kStageShade, 1);
+      hipLaunchKernelGGL(wf_shade<M>, dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
+                         ctl + b, heads + 3 * b + 1, w.queue[(b + 1) & 1], ctl + b + 1, shadow_count + b, b);
+      mark(kStageShade, 0);
+    }
+    mark(kStageShadow, 1);
+    hipLaunchKernelGGL(wf_shadow<M>, dim3(g_shadow), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                       shadow_count + (bounces - 1), heads + 3 * bounces + 2);
+    mark(kStageShadow, 0);
+    mark(kStageEscape, 1);
+    hipLaunchKernelGGL(wf_escape<M>, dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                       w.queue[bounces & 1], ctl + bounces, heads + 3 * bounces);
+    mark(kStageEscape, 0);
+    mark(kStageFilm, 1);
+    hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w,
+                       pass == 0 && p.clear_first ? 1 : 0, pass == p.passes - 1 ? 1 : 0);
+    mark(kStageFilm, 0);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace dfk
+
+hipError_t dfk_launch_pt_wavefront(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k,
+                                   const dfk::DParams& p, const dfk::DFrame& f, const dfk::DPaths& w, int math_mode,
+                                   hipStream_t stream, const dfk_marks* marks) {
+  return math_mode == 1 ? dfk::launch_wavefront<1>(s, c, k, p, f, w, stream, marks)
+                        : dfk::launch_wavefront<0>(s, c, k, p, f, w, stream, marks);
+}

@@ -10,6 +10,9 @@ from .renderer import (  # noqa: F401
     HIPRendererKernel,
     MATH_EXACT,
     MATH_FAST,
+    PT_AUTO,
+    PT_MEGAKERNEL,
+    PT_WAVEFRONT,
     device_count,
 )
 from .scene import (  # noqa: F401

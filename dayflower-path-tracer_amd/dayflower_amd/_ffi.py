@@ -90,8 +90,11 @@ class Params(ctypes.Structure):
         ("global_lacunarity", ctypes.c_float),
         ("global_octaves", ctypes.c_int32),
         ("math_mode", ctypes.c_int32),
+        ("pt_engine", ctypes.c_int32),
     ]
 
+
+STAGE_NAMES = ("megakernel", "raygen", "closest", "shade", "shadow", "escape", "film")
 
 COUNTER_NAMES = ("rays", "samples", "box_tests", "tri_tests", "prim_tests", "sphere_tests", "terrain_steps",
                  "sky_evals")
@@ -139,6 +142,9 @@ HIP_PROTOTYPES = {
     "df_set_film": (ctypes.c_int, [_P, c_float_p, ctypes.c_size_t, c_int32_p, ctypes.c_size_t]),
     "df_get_counters": (ctypes.c_int, [_P, ctypes.POINTER(Counters)]),
     "df_reset_counters": (ctypes.c_int, [_P]),
+    "df_set_stage_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "df_get_stage_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(Counters),
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "df_last_error": (ctypes.c_char_p, [_P]),
 }
 

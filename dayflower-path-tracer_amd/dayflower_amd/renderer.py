@@ -35,6 +35,9 @@ TONE_MAPPER_TYPE_REINHARD_MODIFIED_2 = 4
 
 MATH_FAST = 0
 MATH_EXACT = 1
+PT_AUTO = 0
+PT_MEGAKERNEL = 1
+PT_WAVEFRONT = 2
 
 
 class HIPError(RuntimeError):
@@ -228,6 +231,23 @@ class HIPRendererKernel:
     def resetCounters(self):
         self._check(self._lib.df_reset_counters(self._ctx), "df_reset_counters")
 
+    def setStageTiming(self, enabled):
+        """Brackets every kernel launch with HIP events (see stageStats)."""
+        self._check(self._lib.df_set_stage_timing(self._ctx, 1 if enabled else 0), "df_set_stage_timing")
+
+    def stageStats(self):
+        """{stage: {"counters": {...}, "ms": summed kernel time, "launches": n}} since resetCounters()."""
+        out = {}
+        for i, name in enumerate(_ffi.STAGE_NAMES):
+            c = _ffi.Counters()
+            ms = ctypes.c_double()
+            n = ctypes.c_int64()
+            self._check(self._lib.df_get_stage_stats(self._ctx, i, ctypes.byref(c), ctypes.byref(ms), ctypes.byref(n)),
+                        "df_get_stage_stats")
+            out[name] = {"counters": {k: getattr(c, k) for k in _ffi.COUNTER_NAMES}, "ms": ms.value,
+                         "launches": n.value}
+        return out
+
     # ------------------------------------------------------------ setters (AbstractRendererKernel.java:392-575)
     # Invalid enum values are ignored, as in the reference.
     def setRendererType(self, rendererType):
@@ -303,6 +323,11 @@ class HIPRendererKernel:
     def setMathMode(self, mode):
         """MATH_FAST (device float library) or MATH_EXACT (double-rounded transcendentals)."""
         self._params.math_mode = int(mode)
+        self._push_params()
+
+    def setPathTracerEngine(self, engine):
+        """PT_AUTO, PT_MEGAKERNEL or PT_WAVEFRONT: how the path tracer is scheduled (same results)."""
+        self._params.pt_engine = int(engine)
         self._push_params()
 
     def setParams(self, params):

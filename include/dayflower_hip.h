@@ -126,6 +126,16 @@ int df_set_film(df_ctx* ctx, const float* rgb, size_t rgb_n, const int32_t* samp
 int df_get_counters(df_ctx* ctx, df_counters* out);
 int df_reset_counters(df_ctx* ctx);
 
+/* Per-stage instrumentation (no counterpart in the reference, whose kernel
+ * is one Aparapi launch).  With timing on, every kernel launch is bracketed
+ * by HIP events on the handle's stream; df_get_stage_stats returns, for one
+ * DF_STAGE_*, the counters its kernels accumulated, the summed kernel time
+ * in milliseconds and the number of launches since the last
+ * df_reset_counters.  It synchronizes the stream. */
+int df_set_stage_timing(df_ctx* ctx, int32_t enabled);
+int df_get_stage_stats(df_ctx* ctx, int32_t stage, df_counters* counters, double* milliseconds,
+                       int64_t* launches);
+
 /* Message of the last failed call on this handle (never NULL). */
 const char* df_last_error(df_ctx* ctx);
 

@@ -58,6 +58,28 @@ enum { DF_SHADER_FLAT = 1, DF_SHADER_GOURAUD = 2 };
  *                 parity runs; every other operation is identical in both. */
 enum { DF_MATH_FAST = 0, DF_MATH_EXACT = 1 };
 
+/* How the path tracer is scheduled on the GPU; every choice gives the same
+ * bits.  AUTO picks the wavefront kernels for frames of at least 2^18 pixels
+ * and the megakernel below that.
+ * DF_PT_MEGAKERNEL : one work-item runs a pixel's whole path.
+ * DF_PT_WAVEFRONT  : a kernel per stage (closest-hit, shade, shadow) over
+ *                    queues of live paths. */
+enum { DF_PT_AUTO = 0, DF_PT_MEGAKERNEL = 1, DF_PT_WAVEFRONT = 2 };
+
+/* Kernel stages, for per-stage counters and timings (df_get_stage_stats).
+ * MEGAKERNEL covers every renderer run as one kernel; the others are the
+ * wavefront path tracer's kernels. */
+enum {
+  DF_STAGE_MEGAKERNEL = 0,
+  DF_STAGE_RAYGEN = 1,
+  DF_STAGE_CLOSEST = 2,   /* closest-hit queries of live paths          */
+  DF_STAGE_SHADE = 3,     /* hit record, textures, sun sample, BSDF      */
+  DF_STAGE_SHADOW = 4,    /* any-hit queries towards the sun            */
+  DF_STAGE_ESCAPE = 5,    /* any-hit + sky after the maximum depth      */
+  DF_STAGE_FILM = 6,      /* film update, wireframes, image pass        */
+  DF_STAGE_COUNT = 7
+};
+
 /* The 14 flat arrays of a compiled scene plus the camera, as (pointer,
  * element count) pairs, in CompiledScene file order.  The data is borrowed
  * for the duration of the call that receives it and copied. */
@@ -119,13 +141,17 @@ typedef struct df_params {
   float global_lacunarity;
   int32_t global_octaves;
   int32_t math_mode;
+  int32_t pt_engine;        /* DF_PT_* */
 } df_params;
 
 /* Work counters.  A ray is one intersection query (closest-hit or any-hit);
  * a sample is one film update of one pixel.  The remaining fields count the
  * algorithmic units of the intersection and sky work: BVH box tests,
- * triangle tests, primitive ray transforms, sphere tests, terrain march
- * steps and sky evaluations (the Perez model of an escaped ray). */
+ * triangle tests, primitive ray transforms, sphere tests, terrain height
+ * evaluations and sky evaluations (the Perez model of an escaped ray).
+ * The ray count is the reference algorithm's; the unit counts are the work
+ * this implementation did (it skips terrain heights and queries whose
+ * outcome is already decided). */
 typedef struct df_counters {
   uint64_t rays;
   uint64_t samples;

@@ -60,15 +60,32 @@ def _oracle_render(scene, params, passes, seeds, clear=False, pixel_begin=0, pix
 
 
 CASES = [(1, 96, 96), (2, 128, 96), (3, 160, 90)]
+# (renderer, path-tracer engine): the path tracer runs under both schedules
+RENDERER_ENGINES = [(1, dfa.PT_AUTO), (2, dfa.PT_MEGAKERNEL), (2, dfa.PT_WAVEFRONT), (3, dfa.PT_AUTO),
+                    (4, dfa.PT_AUTO), (5, dfa.PT_AUTO), (6, dfa.PT_AUTO)]
+WORK_COUNTERS = ("box_tests", "tri_tests", "prim_tests", "sphere_tests", "terrain_steps")
 
 
-@pytest.mark.parametrize("renderer", [1, 2, 3, 4, 5, 6])
+def _check_counters(gc, oc):
+    """Rays, samples and sky evaluations are the reference algorithm's; the
+    unit counts are the work done, never more than the oracle's (the device
+    skips terrain heights and the Russian-roulette re-test, whose outcome is
+    decided)."""
+    for name in ("rays", "samples", "sky_evals"):
+        assert gc[name] == oc[name], name
+    for name in WORK_COUNTERS:
+        assert gc[name] <= oc[name], name
+        assert (gc[name] > 0) == (oc[name] > 0), name
+
+
+@pytest.mark.parametrize("renderer,engine", RENDERER_ENGINES)
 @pytest.mark.parametrize("cid,w,h", CASES)
-def test_exact_mode_is_bit_exact(scenes, gpu, cid, w, h, renderer):
+def test_exact_mode_is_bit_exact(scenes, gpu, cid, w, h, renderer, engine):
     sc = _small(scenes(cid), w, h)
     p = sc.copy_params()
     p.renderer_type = renderer
     p.math_mode = dfa.MATH_EXACT
+    p.pt_engine = engine
     passes = 2
     g = _gpu_render(sc, p, passes)
     o = _oracle_render(sc, p, passes, g["seeds0"])
@@ -79,11 +96,60 @@ def test_exact_mode_is_bit_exact(scenes, gpu, cid, w, h, renderer):
     np.testing.assert_array_equal(g["imgf"], o.image_float)
     np.testing.assert_array_equal(g["bgra"], o.image_bgra)
     np.testing.assert_array_equal(g["seeds"], o.seeds)
-    for name in ("rays", "samples", "box_tests", "tri_tests", "prim_tests", "sphere_tests", "sky_evals"):
-        assert g["counters"][name] == o.counters[name], name
-    # the device march evaluates only the heights that can decide the result
-    assert g["counters"]["terrain_steps"] <= o.counters["terrain_steps"]
-    assert (g["counters"]["terrain_steps"] > 0) == (o.counters["terrain_steps"] > 0)
+    _check_counters(g["counters"], o.counters)
+
+
+@pytest.mark.parametrize("cid", [2, 3])
+def test_engines_agree_bit_for_bit_at_full_size(scenes, gpu, cid):
+    """Fast math, full workload resolution: the wavefront schedule gives the
+    megakernel's film, image, seeds and primitive offsets exactly."""
+    sc = scenes(cid)
+    out = []
+    for engine in (dfa.PT_MEGAKERNEL, dfa.PT_WAVEFRONT):
+        p = sc.copy_params()
+        p.math_mode = dfa.MATH_FAST
+        p.pt_engine = engine
+        out.append(_gpu_render(sc, p, 2))
+    a, b = out
+    np.testing.assert_array_equal(a["film"][0], b["film"][0])
+    np.testing.assert_array_equal(a["film"][1], b["film"][1])
+    np.testing.assert_array_equal(a["bgra"], b["bgra"])
+    np.testing.assert_array_equal(a["seeds"], b["seeds"])
+    np.testing.assert_array_equal(a["prims"], b["prims"])
+    for name in ("rays", "samples", "sky_evals") + WORK_COUNTERS:
+        assert a["counters"][name] == b["counters"][name], name
+
+
+def test_wavefront_wireframes_clear_and_resume(scenes, gpu):
+    """Wireframe overlay, a cleared film and a resumed film under the
+    wavefront schedule, against the oracle (exact math)."""
+    sc = _small(scenes(2), 128, 96)
+    p = sc.copy_params()
+    p.math_mode = dfa.MATH_EXACT
+    p.pt_engine = dfa.PT_WAVEFRONT
+    p.wireframes = 1
+    g = _gpu_render(sc, p, 3, clear=True)
+    o = _oracle_render(sc, p, 3, g["seeds0"], clear=True)
+    np.testing.assert_array_equal(g["film"][0], o.film)
+    np.testing.assert_array_equal(g["film"][1], o.film_samples)
+    np.testing.assert_array_equal(g["bgra"], o.image_bgra)
+    _check_counters(g["counters"], o.counters)
+    # continue from the oracle's film and seeds for two more passes
+    k = dfa.HIPRendererKernel(sc)
+    try:
+        k.setParams(p)
+        k.update(sc.width, sc.height)
+        k.setSeeds(o.seeds)
+        k.setFilm(o.film, o.film_samples)
+        k.execute(2)
+        film2, n2 = k.getFilm()
+        seeds2 = k.getSeeds()
+    finally:
+        k.dispose()
+    oracle_ffi.render(sc, p, o, 2)
+    np.testing.assert_array_equal(film2, o.film)
+    np.testing.assert_array_equal(n2, o.film_samples)
+    np.testing.assert_array_equal(seeds2, o.seeds)
 
 
 def test_exact_mode_wireframes_and_flat_shading(scenes, gpu):
