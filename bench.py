@@ -96,6 +96,9 @@ def main():
     ap.add_argument("--math", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain 1 GiB float copy kernel (negation), the reference for scaling the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -110,6 +113,16 @@ def main():
     if not torch.cuda.is_available():
         log("no HIP device visible")
         sys.exit(1)
+    if args.calibrate_copy:
+        n = 1 << 28  # 1 GiB of float32 read and 1 GiB written per launch
+        x = torch.ones(n, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.neg(x, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "torch.neg float32", "bytes_read_per_launch": n * 4,
+                          "bytes_written_per_launch": n * 4, "launches": 3}), flush=True)
+        return
 
     scene = dfa.build_config(args.workload)
     params = scene.copy_params()

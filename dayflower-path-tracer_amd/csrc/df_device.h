@@ -32,16 +32,32 @@ namespace dfk {
 
 // ------------------------------------------------------------ Java semantics
 
-__device__ __forceinline__ float jmax(float a, float b) {  // Math.max(float, float)
-  if (a != a) return a;
-  if (a == 0.0f && b == 0.0f && __builtin_signbit(a)) return b;
-  return a >= b ? a : b;
+// p[i] of scene data, which no kernel writes: read through the constant
+// address space, a wave-uniform index becomes scalar loads.  (The host
+// compilation pass of this header has no such address space.)
+template <class T>
+__device__ __forceinline__ T load_uniform(const T* p, int i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(4))) T*)(p))[i];
+#else
+  return p[i];
+#endif
 }
-__device__ __forceinline__ float jmin(float a, float b) {  // Math.min(float, float)
-  if (a != a) return a;
-  if (a == 0.0f && b == 0.0f && __builtin_signbit(b)) return b;
-  return a <= b ? a : b;
+
+// p[i] with a 32-bit byte offset (arrays here stay far below 4 GiB), so the
+// load takes the scalar-base + 32-bit-offset form instead of 64-bit address
+// arithmetic per access.
+template <class T>
+__device__ __forceinline__ T at(const T* p, int i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(p) +
+                                     static_cast<uint32_t>(i) * static_cast<uint32_t>(sizeof(T)));
 }
+
+// Math.max / Math.min (float): NaN if either operand is NaN, -0 below +0 -
+// the IEEE 754-2019 maximum / minimum, one v_maximum3_f32 / v_minimum3_f32
+// on gfx950 (nested calls fuse into the three-operand form).
+__device__ __forceinline__ float jmax(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+__device__ __forceinline__ float jmin(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 __device__ __forceinline__ double jmaxd(double a, double b) {
   if (a != a) return a;
   if (a == 0.0 && b == 0.0 && __builtin_signbit(a)) return b;

@@ -130,7 +130,7 @@ struct Tracer {
     cso = -1;
     const int n = S.prim_count;
     for (int i = 0; i < n; ++i) {
-      const DPrim& pr = S.prims[i];
+      const DPrim pr = load_uniform(S.prims, i);  // wave-uniform: scalar loads
       const int st = pr.type;
       ++prim_tests;
       const float* m = pr.w2o;
@@ -145,7 +145,7 @@ struct Tracer {
         const float rdx = 1.0f / dxo, rdy = 1.0f / dyo, rdz = 1.0f / dzo;
         int idx = pr.node_root;
         while (idx != -1) {
-          const DNode nd = S.nodes[idx];
+          const DNode nd = at(S.nodes, idx);
           ++box_tests;
           const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
           const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
@@ -162,7 +162,7 @@ struct Tracer {
           } else {
             int k = nd.tri;
             for (;;) {
-              const DTri tr = S.tris[k];
+              const DTri tr = at(S.tris, k);
               ++tri_tests;
               cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
               if (cur < closest) {
