@@ -44,6 +44,8 @@ FLOP_PER = {"box_tests": 24, "tri_tests": 45, "prim_tests": 36, "sphere_tests": 
             "sky_evals": 60}
 BYTES_PER = {"box_tests": 32, "tri_tests": 36, "prim_tests": 48, "sphere_tests": 16}
 PEAK_FP32_TFLOPS = 157.3
+KERNEL_SYMBOLS = {"megakernel": "dfk::render_kernel", "raygen": "dfk::wf_raygen", "closest": "dfk::wf_closest",
+                  "shade": "dfk::wf_shade", "shadow": "dfk::wf_shadow", "escape": "dfk::wf_escape", "film": "dfk::wf_film"}
 KERNEL_NAMES = {"megakernel": "render_kernel (one work-item per pixel)", "raygen": "wf_raygen",
                 "closest": "wf_closest (closest-hit queries)", "shade": "wf_shade", "shadow": "wf_shadow (sun any-hit)",
                 "escape": "wf_escape", "film": "wf_film"}
@@ -87,6 +89,55 @@ def cpu_baseline(scene, params, seed, target_seconds, threads):
     }, np.float64(dt)
 
 
+def _pmc_per_kernel(path):
+    """{kernel name (no arguments): [counter value per dispatch]} from a rocprofv3 counter_collection.csv."""
+    import csv
+
+    out = {}
+    for r in csv.DictReader(open(path)):
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+        out.setdefault(name, []).append(float(r["Counter_Value"]))
+    return out
+
+
+def traffic_summary(out_path, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv, workload):
+    """HBM bytes per dispatch of each renderer kernel: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each
+    scaled by bytes / counter of the 1 GiB calibration copy (which reads and writes 2^30 bytes)."""
+    gib_kib = float(1 << 20)
+    cal_f = max(v for vals in _pmc_per_kernel(cal_fetch_csv).values() for v in vals)
+    cal_w = max(v for vals in _pmc_per_kernel(cal_write_csv).values() for v in vals)
+    f_scale, w_scale = gib_kib / cal_f, gib_kib / cal_w
+    fetch, write = _pmc_per_kernel(fetch_csv), _pmc_per_kernel(write_csv)
+    kernels = {}
+    for name in fetch:
+        if not name.startswith("dfk::") or name not in write:
+            continue
+        f = sum(fetch[name]) / len(fetch[name])
+        w = sum(write[name]) / len(write[name])
+        kernels[name] = {"dispatches": len(fetch[name]), "fetch_kib": round(f, 1), "write_kib": round(w, 1),
+                         "hbm_bytes_per_dispatch": round((f * f_scale + w * w_scale) * 1024.0)}
+    summary = {"workload": workload, "fetch_scale": round(f_scale, 4), "write_scale": round(w_scale, 4),
+               "calibration": "torch.neg over 2^28 float32 (2^30 bytes read, 2^30 written): FETCH_SIZE %.1f KiB, "
+                              "WRITE_SIZE %.1f KiB per dispatch" % (cal_f, cal_w),
+               "kernels": kernels}
+    with open(out_path, "w") as fh:
+        json.dump(summary, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps(summary))
+
+
+def traffic_from_profiles(workload, kernel_prefix):
+    """HBM bytes per dispatch of the kernel from profiles/traffic_workload<N>.json, or None."""
+    path = os.path.join(REPO, "profiles", "traffic_workload%d.json" % workload)
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    for name, k in data.get("kernels", {}).items():
+        if name.split("<")[0] == kernel_prefix:
+            return k["hbm_bytes_per_dispatch"]
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -99,7 +150,14 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB float copy kernel (negation), the reference for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--traffic-summary", nargs=5, metavar=("OUT", "FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV",
+                                                           "CAL_WRITE_CSV"),
+                    help="write the HBM-traffic summary OUT from rocprofv3 --pmc counter_collection.csv files of "
+                         "this bench (FETCH_SIZE, WRITE_SIZE) and of --calibrate-copy (same counters)")
     args = ap.parse_args()
+    if args.traffic_summary:
+        traffic_summary(*args.traffic_summary, workload=args.workload)
+        return
 
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -170,7 +228,7 @@ def main():
         "peak": PEAK_FP32_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 6),
-        "traffic": None,
+        "traffic": traffic_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, "")),
         "kernel": KERNEL_NAMES[dom],
         "note": "VALU-bound (no matrix work): peak is the FP32 figure; algorithmic bytes %.4g GB/s = %.4f of HBM"
                 % (achieved_gbs, achieved_gbs / PEAK_HBM_GBS),
