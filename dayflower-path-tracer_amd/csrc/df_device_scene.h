@@ -138,6 +138,24 @@ struct DFrame {
 
 };
 
+// Everything one scene of a batch needs: its scene, camera, sky, parameters
+// and frame (the arguments of render_kernel).
+struct DJob {
+  DScene S;
+  DCamera C;
+  DSky K;
+  DParams P;
+  DFrame F;
+};
+
+// One work item of a batch launch: a 16x16 tile of a job, all its passes.
+struct DBatchItem {
+  int32_t job;
+  int32_t tile_x;
+  int32_t tile_y;
+  int32_t pad;
+};
+
 // Path state of the wavefront path tracer, one slot per pixel (y * W + x),
 // and its work queues.  float4 records keep each slot one 16-byte access.
 struct DPaths {

@@ -16,6 +16,11 @@ hipError_t dfk_launch_render(const dfk::DScene& s, const dfk::DCamera& c, const 
                              const dfk::DFrame& f, int math_mode, hipStream_t stream,
                              const dfk_marks* marks = nullptr);
 
+// Persistent launch over a batch's (job, tile) items of one math mode and
+// renderer type; head must be zero.
+hipError_t dfk_launch_batch(const dfk::DJob* jobs, const dfk::DBatchItem* items, int n_items, int* head,
+                            int math_mode, int renderer_type, hipStream_t stream);
+
 hipError_t dfk_launch_seed(uint64_t* seeds, uint64_t n, uint64_t master, hipStream_t stream);
 
 // One or more passes of the path tracer as wavefront kernels (df_wavefront.hip);

@@ -13,78 +13,117 @@
 
 namespace dfk {
 
-// M: transcendental mode; R: renderer type (1 AO, 2 path tracer, 3 ray caster,
-// 4 ray marcher, 5 ray tracer, 6 surface normals), a template parameter so
-// that each instantiation holds only its own renderer's ray queries.
+// All `P.passes` passes of run() for pixel (x, y): film, sample count, seed
+// and primitive offset stay in registers and are written once, followed by
+// the tone-mapped pixel.  R is the renderer type (1 AO, 2 path tracer, 3 ray
+// caster, 4 ray marcher, 5 ray tracer, 6 surface normals), a template
+// parameter so that each instantiation holds only its own ray queries.
+template <int M, int R>
+__device__ __forceinline__ void render_pixel(const DScene& S, const DCamera& C, const DSky& K, const DParams& P,
+                                             const DFrame& F, const int* perm, int x, int y,
+                                             unsigned long long cnt[kCounterCount]) {
+  const int gid = y * P.width + x;
+  Tracer<M> tr(S, C, K, P, perm);
+  tr.rng.s = F.seeds[gid];
+  float film[3] = {F.film[gid * 3 + 0], F.film[gid * 3 + 1], F.film[gid * 3 + 2]};
+  int nsamp = F.film_samples[gid];
+  int prim_off = F.primitive_offsets[gid];
+  const bool sky_active = K.sky_active == 1;
+  const bool sun_active = K.sun_active == 1;
+  for (int pass = 0; pass < P.passes; ++pass) {
+    const bool clear = pass == 0 && P.clear_first;
+    float ray[6];
+    float col[3] = {0.0f, 0.0f, 0.0f};
+    bool set = false;  // filmSetColor instead of filmAddColor
+    if (tr.primary_ray(x, y, ray)) {
+      if constexpr (R == 1) {
+        tr.render_ao(ray, col, &prim_off);
+      } else if constexpr (R == 3) {
+        tr.render_rc(ray, sky_active, col, &prim_off);
+      } else if constexpr (R == 4) {
+        tr.render_rm(ray, sky_active, col);
+      } else if constexpr (R == 5) {
+        tr.render_rt(ray, sky_active, col, &prim_off);
+      } else if constexpr (R == 6) {
+        tr.render_sn(ray, sky_active, col, &prim_off);
+      } else {
+        tr.render_pt(ray, sky_active, sun_active, col, &prim_off);
+      }
+    } else {
+      set = true;
+    }
+    // filmAddColor / filmSetColor (AbstractImageKernel.java:333-378)
+    ++tr.samples;
+    if (set || clear) {
+      film[0] = col[0];
+      film[1] = col[1];
+      film[2] = col[2];
+      nsamp = 1;
+    } else {
+      const int n = nsamp + 1;
+      film[0] = film[0] + ((col[0] - film[0]) / static_cast<float>(n));
+      film[1] = film[1] + ((col[1] - film[1]) / static_cast<float>(n));
+      film[2] = film[2] + ((col[2] - film[2]) / static_cast<float>(n));
+      nsamp = n;
+    }
+    if (!set && P.wireframes == 1 && tr.wireframe_edge(ray)) {
+      ++tr.samples;
+      film[0] = film[1] = film[2] = 0.0f;
+      nsamp = 1;
+    }
+  }
+  F.seeds[gid] = tr.rng.s;
+  F.film[gid * 3 + 0] = film[0];
+  F.film[gid * 3 + 1] = film[1];
+  F.film[gid * 3 + 2] = film[2];
+  F.film_samples[gid] = nsamp;
+  F.primitive_offsets[gid] = prim_off;
+  image_pass<M>(P, film, prim_off, F.image_float + gid * 3, F.image_bgra + gid * 4);
+  take_counters(tr, cnt);
+}
+
+__device__ __forceinline__ void stage_perm_2d(int* perm) {
+  for (int i = threadIdx.y * 16 + threadIdx.x; i < 512; i += 256) perm[i] = kPerm[i & 255];
+  __syncthreads();
+}
+
 template <int M, int R>
 __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K, DParams P, DFrame F) {
   __shared__ int perm[512];
-  for (int i = threadIdx.y * 16 + threadIdx.x; i < 512; i += 256) perm[i] = kPerm[i & 255];
-  __syncthreads();
+  stage_perm_2d(perm);
   const int x = blockIdx.x * 16 + threadIdx.x;
   const int y = blockIdx.y * 16 + threadIdx.y;
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (x < P.width && y < P.height) {
-    const int gid = y * P.width + x;
-    Tracer<M> tr(S, C, K, P, perm);
-    tr.rng.s = F.seeds[gid];
-    float film[3] = {F.film[gid * 3 + 0], F.film[gid * 3 + 1], F.film[gid * 3 + 2]};
-    int nsamp = F.film_samples[gid];
-    int prim_off = F.primitive_offsets[gid];
-    const bool sky_active = K.sky_active == 1;
-    const bool sun_active = K.sun_active == 1;
-    for (int pass = 0; pass < P.passes; ++pass) {
-      const bool clear = pass == 0 && P.clear_first;
-      float ray[6];
-      float col[3] = {0.0f, 0.0f, 0.0f};
-      bool set = false;  // filmSetColor instead of filmAddColor
-      if (tr.primary_ray(x, y, ray)) {
-        if constexpr (R == 1) {
-          tr.render_ao(ray, col, &prim_off);
-        } else if constexpr (R == 3) {
-          tr.render_rc(ray, sky_active, col, &prim_off);
-        } else if constexpr (R == 4) {
-          tr.render_rm(ray, sky_active, col);
-        } else if constexpr (R == 5) {
-          tr.render_rt(ray, sky_active, col, &prim_off);
-        } else if constexpr (R == 6) {
-          tr.render_sn(ray, sky_active, col, &prim_off);
-        } else {
-          tr.render_pt(ray, sky_active, sun_active, col, &prim_off);
-        }
-      } else {
-        set = true;
-      }
-      // filmAddColor / filmSetColor (AbstractImageKernel.java:333-378)
-      ++tr.samples;
-      if (set || clear) {
-        film[0] = col[0];
-        film[1] = col[1];
-        film[2] = col[2];
-        nsamp = 1;
-      } else {
-        const int n = nsamp + 1;
-        film[0] = film[0] + ((col[0] - film[0]) / static_cast<float>(n));
-        film[1] = film[1] + ((col[1] - film[1]) / static_cast<float>(n));
-        film[2] = film[2] + ((col[2] - film[2]) / static_cast<float>(n));
-        nsamp = n;
-      }
-      if (!set && P.wireframes == 1 && tr.wireframe_edge(ray)) {
-        ++tr.samples;
-        film[0] = film[1] = film[2] = 0.0f;
-        nsamp = 1;
-      }
-    }
-    F.seeds[gid] = tr.rng.s;
-    F.film[gid * 3 + 0] = film[0];
-    F.film[gid * 3 + 1] = film[1];
-    F.film[gid * 3 + 2] = film[2];
-    F.film_samples[gid] = nsamp;
-    F.primitive_offsets[gid] = prim_off;
-    image_pass<M>(P, film, prim_off, F.image_float + gid * 3, F.image_bgra + gid * 4);
-    take_counters(tr, cnt);
-  }
+  if (x < P.width && y < P.height) render_pixel<M, R>(S, C, K, P, F, perm, x, y, cnt);
   flush_counters(cnt, F.counters + kStageMega * kCounterCount);
+}
+
+// A batch of independent scenes in one persistent launch: each workgroup
+// takes (job, tile) items from an atomic head - the items ordered by
+// decreasing estimated cost - and renders all of the tile's passes.  A pixel's
+// passes stay in one work-item, in its own RNG stream, so every job's result
+// is the one render_kernel gives for that scene alone.
+template <int M, int R>
+__global__ __launch_bounds__(256) void batch_kernel(const DJob* __restrict__ jobs,
+                                                    const DBatchItem* __restrict__ items, int n_items, int* head) {
+  __shared__ int perm[512];
+  __shared__ int next;
+  stage_perm_2d(perm);
+  for (;;) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) next = atomicAdd(head, 1);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(next);
+    __syncthreads();  // every work-item has read `next` before it is replaced
+    if (it >= n_items) break;
+    const DBatchItem item = items[it];
+    // jobs is read-only and restrict: its fields are scalar loads, as kernel arguments are
+    const DJob& J = jobs[item.job];
+    const int x = item.tile_x * 16 + threadIdx.x;
+    const int y = item.tile_y * 16 + threadIdx.y;
+    unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (x < J.P.width && y < J.P.height) render_pixel<M, R>(J.S, J.C, J.K, J.P, J.F, perm, x, y, cnt);
+    flush_counters(cnt, J.F.counters + kStageMega * kCounterCount);
+  }
 }
 
 // Per-pixel seeds from a master seed: SplitMix64 output for state
@@ -116,6 +155,31 @@ hipError_t dfk_launch_render(const dfk::DScene& s, const dfk::DCamera& c, const 
   if (marks) marks->mark(marks->user, dfk::kStageMega, 1);
   hipLaunchKernelGGL((math_mode == 1 ? kExact : kFast)[r], grid, block, 0, stream, s, c, k, p, f);
   if (marks) marks->mark(marks->user, dfk::kStageMega, 0);
+  return hipGetLastError();
+}
+
+hipError_t dfk_launch_batch(const dfk::DJob* jobs, const dfk::DBatchItem* items, int n_items, int* head,
+                            int math_mode, int renderer_type, hipStream_t stream) {
+  using Fn = void (*)(const dfk::DJob* __restrict__, const dfk::DBatchItem* __restrict__, int, int*);
+  static const Fn kFast[6] = {dfk::batch_kernel<0, 1>, dfk::batch_kernel<0, 2>, dfk::batch_kernel<0, 3>,
+                              dfk::batch_kernel<0, 4>, dfk::batch_kernel<0, 5>, dfk::batch_kernel<0, 6>};
+  static const Fn kExact[6] = {dfk::batch_kernel<1, 1>, dfk::batch_kernel<1, 2>, dfk::batch_kernel<1, 3>,
+                               dfk::batch_kernel<1, 4>, dfk::batch_kernel<1, 5>, dfk::batch_kernel<1, 6>};
+  const int r = renderer_type >= 1 && renderer_type <= 6 ? renderer_type - 1 : 1;
+  const Fn fn = (math_mode == 1 ? kExact : kFast)[r];
+  static int grid[2][6] = {};
+  int& g = grid[math_mode == 1 ? 1 : 0][r];
+  if (g == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    g = cus * per_cu;
+  }
+  if (n_items <= 0) return hipSuccess;
+  const int blocks = n_items < g ? n_items : g;
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(16, 16), 0, stream, jobs, items, n_items, head);
   return hipGetLastError();
 }
 

@@ -20,8 +20,10 @@
 #include "df_device_scene.h"
 #include "df_kernels.h"
 
+using dfk::DBatchItem;
 using dfk::DCamera;
 using dfk::DFrame;
+using dfk::DJob;
 using dfk::DNode;
 using dfk::DParams;
 using dfk::DPrim;
@@ -714,6 +716,61 @@ int read_back(df_ctx* c, void* dst, const DevBuf& b, size_t bytes) {
   return DF_OK;
 }
 
+DFrame frame_of(df_ctx* c) {
+  DFrame f{};
+  f.film = static_cast<float*>(c->film.p);
+  f.film_samples = static_cast<int32_t*>(c->film_samples.p);
+  f.image_float = static_cast<float*>(c->image_float.p);
+  f.image_bgra = static_cast<uint8_t*>(c->image_bgra.p);
+  f.primitive_offsets = static_cast<int32_t*>(c->prim_offsets.p);
+  f.seeds = static_cast<uint64_t*>(c->seeds.p);
+  f.counters = static_cast<unsigned long long*>(c->counters.p);
+  return f;
+}
+
+}  // namespace
+
+struct df_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  std::string err;
+  std::vector<df_ctx*> jobs;
+  std::vector<int32_t> passes;
+  DevBuf djobs, ditems, dheads;
+};
+
+namespace {
+
+int bfail(df_batch* b, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+#define DF_BHIP(b, call)                                                             \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess) return bfail((b), DF_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+int balloc(df_batch* b, DevBuf& d, size_t bytes) {
+  free_buf(d);
+  if (bytes == 0) bytes = 4;
+  const hipError_t e = hipMalloc(&d.p, bytes);
+  if (e != hipSuccess) {
+    d.p = nullptr;
+    return bfail(b, DF_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  d.bytes = bytes;
+  return DF_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -996,14 +1053,7 @@ int df_execute(df_ctx* c, int32_t passes) {
   const DCamera cam = make_camera(c->camera);
   const DSky sky = make_sky(c->sky);
   const DParams par = make_params(c->params, c->width, c->height, passes, c->clear_pending);
-  DFrame f{};
-  f.film = static_cast<float*>(c->film.p);
-  f.film_samples = static_cast<int32_t*>(c->film_samples.p);
-  f.image_float = static_cast<float*>(c->image_float.p);
-  f.image_bgra = static_cast<uint8_t*>(c->image_bgra.p);
-  f.primitive_offsets = static_cast<int32_t*>(c->prim_offsets.p);
-  f.seeds = static_cast<uint64_t*>(c->seeds.p);
-  f.counters = static_cast<unsigned long long*>(c->counters.p);
+  const DFrame f = frame_of(c);
   const bool wavefront = use_wavefront(c);
   if (wavefront && (rc = ensure_paths(c, c->params.pt_depth_maximum))) return rc;
   const dfk_marks marks{stage_mark, c};
@@ -1132,6 +1182,148 @@ int df_get_stage_stats(df_ctx* c, int32_t stage, df_counters* counters, double* 
 const char* df_last_error(df_ctx* c) {
   if (!c) return "null handle";
   return c->err.c_str();
+}
+
+// ------------------------------------------------------------ batches of scenes
+
+int df_batch_create(df_batch** out, int32_t device) {
+  if (!out) return DF_EINVAL;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DF_ENODEVICE;
+  if (device < 0 || device >= n) return DF_ENODEVICE;
+  df_batch* b = new (std::nothrow) df_batch;
+  if (!b) return DF_ENOMEM;
+  b->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    df_batch_destroy(b);
+    return DF_EHIP;
+  }
+  *out = b;
+  return DF_OK;
+}
+
+void df_batch_destroy(df_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (df_ctx* c : b->jobs) df_destroy(c);
+  free_buf(b->djobs);
+  free_buf(b->ditems);
+  free_buf(b->dheads);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int df_batch_add(df_batch* b, const df_compiled_scene* scene, const df_sky* sky, const df_params* params,
+                 int32_t width, int32_t height, int32_t passes, uint64_t seed, int32_t* index) {
+  if (!b || !scene || !sky || !params) return DF_EINVAL;
+  if (passes < 0) return bfail(b, DF_EINVAL, "passes %d", passes);
+  df_ctx* c = nullptr;
+  int rc = df_create(&c, b->device);
+  if (rc) return bfail(b, rc, "df_create failed (%d)", rc);
+  if ((rc = df_load_scene(c, scene)) || (rc = df_upload_sky(c, sky)) || (rc = df_set_params(c, params)) ||
+      (rc = df_update(c, width, height, 0)) || (rc = df_seed(c, seed))) {
+    bfail(b, rc, "job %zu: %s", b->jobs.size(), df_last_error(c));
+    df_destroy(c);
+    return rc;
+  }
+  b->jobs.push_back(c);
+  b->passes.push_back(passes);
+  if (index) *index = static_cast<int32_t>(b->jobs.size() - 1);
+  return DF_OK;
+}
+
+int df_batch_count(df_batch* b, int32_t* count) {
+  if (!b || !count) return DF_EINVAL;
+  *count = static_cast<int32_t>(b->jobs.size());
+  return DF_OK;
+}
+
+int df_batch_job(df_batch* b, int32_t index, df_ctx** ctx) {
+  if (!b || !ctx) return DF_EINVAL;
+  if (index < 0 || index >= static_cast<int32_t>(b->jobs.size())) return bfail(b, DF_EINVAL, "job %d", index);
+  DF_BHIP(b, hipStreamSynchronize(b->stream));
+  *ctx = b->jobs[index];
+  return DF_OK;
+}
+
+int df_batch_execute(df_batch* b) {
+  if (!b) return DF_EINVAL;
+  const size_t n = b->jobs.size();
+  if (n == 0) return DF_OK;
+  DF_BHIP(b, hipSetDevice(b->device));
+  struct Item {
+    double cost;
+    int group;
+    DBatchItem it;
+  };
+  std::vector<DJob> hj(n);
+  std::vector<Item> items;
+  std::vector<int> group_key;  // math_mode * 8 + renderer_type, one entry per group
+  for (size_t j = 0; j < n; ++j) {
+    df_ctx* c = b->jobs[j];
+    DF_BHIP(b, hipStreamSynchronize(c->stream));
+    hj[j].S = c->ds;
+    hj[j].C = make_camera(c->camera);
+    hj[j].K = make_sky(c->sky);
+    hj[j].P = make_params(c->params, c->width, c->height, b->passes[j], c->clear_pending);
+    hj[j].F = frame_of(c);
+    if (b->passes[j] == 0) continue;
+    const int key = c->params.math_mode * 8 + c->params.renderer_type;
+    int g = 0;
+    while (g < static_cast<int>(group_key.size()) && group_key[g] != key) ++g;
+    if (g == static_cast<int>(group_key.size())) group_key.push_back(key);
+    // work estimate of a tile: passes x (primitive loop + BVH depth)
+    const double cost = static_cast<double>(b->passes[j]) *
+                        (16.0 + c->ds.prim_count + 8.0 * std::log2(1.0 + static_cast<double>(c->hs.bvh.size())));
+    const int tw = (c->width + 15) / 16, th = (c->height + 15) / 16;
+    for (int ty = 0; ty < th; ++ty)
+      for (int tx = 0; tx < tw; ++tx) items.push_back({cost, g, {static_cast<int32_t>(j), tx, ty, 0}});
+  }
+  // per group, the most expensive items first (longest-processing-time order)
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
+    return a.group != c.group ? a.group < c.group : a.cost > c.cost;
+  });
+  std::vector<DBatchItem> hi(items.size());
+  for (size_t i = 0; i < items.size(); ++i) hi[i] = items[i].it;
+  int rc;
+  if ((rc = balloc(b, b->djobs, n * sizeof(DJob))) || (rc = balloc(b, b->ditems, hi.size() * sizeof(DBatchItem))) ||
+      (rc = balloc(b, b->dheads, (group_key.size() + 1) * sizeof(int))))
+    return rc;
+  DF_BHIP(b, hipMemcpy(b->djobs.p, hj.data(), n * sizeof(DJob), hipMemcpyHostToDevice));
+  if (!hi.empty()) DF_BHIP(b, hipMemcpy(b->ditems.p, hi.data(), hi.size() * sizeof(DBatchItem), hipMemcpyHostToDevice));
+  DF_BHIP(b, hipMemset(b->dheads.p, 0, (group_key.size() + 1) * sizeof(int)));
+  DF_BHIP(b, hipEventRecord(b->ev0, b->stream));
+  size_t begin = 0;
+  for (size_t g = 0; g < group_key.size(); ++g) {
+    size_t end = begin;
+    while (end < items.size() && items[end].group == static_cast<int>(g)) ++end;
+    DF_BHIP(b, dfk_launch_batch(static_cast<const DJob*>(b->djobs.p), static_cast<const DBatchItem*>(b->ditems.p) + begin,
+                                static_cast<int>(end - begin), static_cast<int*>(b->dheads.p) + g, group_key[g] / 8,
+                                group_key[g] % 8, b->stream));
+    begin = end;
+  }
+  DF_BHIP(b, hipEventRecord(b->ev1, b->stream));
+  DF_BHIP(b, hipStreamSynchronize(b->stream));
+  b->timed = true;
+  for (df_ctx* c : b->jobs) c->clear_pending = false;
+  return DF_OK;
+}
+
+int df_batch_last_execute_ms(df_batch* b, float* ms) {
+  if (!b || !ms) return DF_EINVAL;
+  if (!b->timed) return bfail(b, DF_ESTATE, "nothing executed yet");
+  DF_BHIP(b, hipEventElapsedTime(ms, b->ev0, b->ev1));
+  return DF_OK;
+}
+
+const char* df_batch_last_error(df_batch* b) {
+  if (!b) return "null handle";
+  return b->err.c_str();
 }
 
 }  // extern "C"

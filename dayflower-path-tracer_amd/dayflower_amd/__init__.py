@@ -6,6 +6,7 @@
 
 from . import _ffi  # noqa: F401
 from .renderer import (  # noqa: F401
+    HIPBatchRenderer,
     HIPError,
     HIPRendererKernel,
     MATH_EXACT,

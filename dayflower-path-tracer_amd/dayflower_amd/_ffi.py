@@ -146,6 +146,15 @@ HIP_PROTOTYPES = {
     "df_get_stage_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(Counters),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "df_last_error": (ctypes.c_char_p, [_P]),
+    "df_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32]),
+    "df_batch_destroy": (None, [_P]),
+    "df_batch_add": (ctypes.c_int, [_P, ctypes.POINTER(CompiledScene), ctypes.POINTER(Sky), ctypes.POINTER(Params),
+                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64, c_int32_p]),
+    "df_batch_count": (ctypes.c_int, [_P, c_int32_p]),
+    "df_batch_job": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "df_batch_execute": (ctypes.c_int, [_P]),
+    "df_batch_last_execute_ms": (ctypes.c_int, [_P, c_float_p]),
+    "df_batch_last_error": (ctypes.c_char_p, [_P]),
 }
 
 SCENE_PROTOTYPES = {

@@ -86,11 +86,26 @@ class HIPRendererKernel:
         self._check(self._lib.df_set_params(self._ctx, ctypes.byref(self._params)), "df_set_params")
         self._changed = True
 
+    @classmethod
+    def _borrowed(cls, lib, ctx, scene, width, height):
+        """A view of a renderer handle owned elsewhere (a batch job); dispose() leaves it alone."""
+        self = cls.__new__(cls)
+        self._lib = lib
+        self._ctx = ctx
+        self._owned = False
+        self._scene = scene
+        self._image = None
+        self.width, self.height = int(width), int(height)
+        self._changed = False
+        self._params = _ffi.Params()
+        self._check(self._lib.df_get_params(self._ctx, ctypes.byref(self._params)), "df_get_params")
+        return self
+
     def dispose(self):
         """Kernel.dispose(): releases the device resources."""
-        if self._ctx:
+        if self._ctx and getattr(self, "_owned", True):
             self._lib.df_destroy(self._ctx)
-            self._ctx = None
+        self._ctx = None
 
     def __del__(self):
         try:
@@ -364,3 +379,67 @@ class HIPRendererKernel:
 
     def setChanged(self, changed):
         self._changed = bool(changed)
+
+
+class HIPBatchRenderer:
+    """Many independent scenes rendered by one persistent launch (df_batch_*).
+
+    ``add`` registers a scene with its own parameters, resolution, passes per
+    execute and seed; ``execute`` renders every scene's passes; ``job(i)``
+    returns a HIPRendererKernel view of scene i for readbacks (or to render
+    it alone).  Each scene's result equals rendering it alone."""
+
+    def __init__(self, device=0):
+        self._lib = _ffi.hip_lib()
+        self._b = ctypes.c_void_p()
+        rc = self._lib.df_batch_create(ctypes.byref(self._b), int(device))
+        if rc != _ffi.DF_OK:
+            self._b = None
+            raise HIPError(rc, "df_batch_create(device=%d)" % device)
+        self._jobs = []
+
+    def _check(self, rc, what):
+        if rc != _ffi.DF_OK:
+            msg = self._lib.df_batch_last_error(self._b) if self._b else b""
+            raise HIPError(rc, "%s: %s" % (what, (msg or b"").decode()))
+
+    def add(self, scene, params=None, width=None, height=None, passes=1, seed=0):
+        if not isinstance(scene, CompiledScene):
+            raise TypeError("scene must be a CompiledScene")
+        p = scene.copy_params() if params is None else params
+        w = scene.width if width is None else int(width)
+        h = scene.height if height is None else int(height)
+        idx = ctypes.c_int32()
+        self._check(self._lib.df_batch_add(self._b, ctypes.byref(scene.as_struct()), ctypes.byref(scene.sky),
+                                           ctypes.byref(p), w, h, int(passes), ctypes.c_uint64(seed),
+                                           ctypes.byref(idx)), "df_batch_add")
+        self._jobs.append((scene, w, h))
+        return idx.value
+
+    def __len__(self):
+        return len(self._jobs)
+
+    def execute(self):
+        self._check(self._lib.df_batch_execute(self._b), "df_batch_execute")
+
+    def lastExecuteMillis(self):
+        ms = ctypes.c_float()
+        self._check(self._lib.df_batch_last_execute_ms(self._b, ctypes.byref(ms)), "df_batch_last_execute_ms")
+        return ms.value
+
+    def job(self, i):
+        ctx = ctypes.c_void_p()
+        self._check(self._lib.df_batch_job(self._b, int(i), ctypes.byref(ctx)), "df_batch_job")
+        scene, w, h = self._jobs[i]
+        return HIPRendererKernel._borrowed(self._lib, ctx, scene, w, h)
+
+    def dispose(self):
+        if self._b:
+            self._lib.df_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.dispose()
+        except Exception:
+            pass

@@ -139,6 +139,35 @@ int df_get_stage_stats(df_ctx* ctx, int32_t stage, df_counters* counters, double
 /* Message of the last failed call on this handle (never NULL). */
 const char* df_last_error(df_ctx* ctx);
 
+/* ---- Batches of independent scenes ---------------------------------------
+ * The reference renders one scene per kernel; these entry points (planned
+ * with the ABI as df_batch_*) render many scenes in one persistent launch:
+ * workgroups take (scene, 16x16 tile) items from a device queue, most
+ * expensive first, and render all of a tile's passes.  Each scene keeps its
+ * own renderer handle, so its result equals rendering it alone with
+ * df_execute(passes), bit for bit. */
+typedef struct df_batch df_batch;
+
+int df_batch_create(df_batch** out, int32_t device);
+void df_batch_destroy(df_batch* batch);
+
+/* Adds a scene: load, sky, parameters, width x height frame, df_seed(seed);
+ * `passes` samples per pixel per df_batch_execute.  *index = its position. */
+int df_batch_add(df_batch* batch, const df_compiled_scene* scene, const df_sky* sky, const df_params* params,
+                 int32_t width, int32_t height, int32_t passes, uint64_t seed, int32_t* index);
+int df_batch_count(df_batch* batch, int32_t* count);
+
+/* The scene's renderer handle, owned by the batch (readbacks, counters, or a
+ * df_execute of that scene alone).  Do not df_destroy it. */
+int df_batch_job(df_batch* batch, int32_t index, df_ctx** ctx);
+
+/* Renders every scene's passes; returns when the batch is done. */
+int df_batch_execute(df_batch* batch);
+
+/* Device time of the last df_batch_execute in milliseconds. */
+int df_batch_last_execute_ms(df_batch* batch, float* ms);
+const char* df_batch_last_error(df_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,3 +57,11 @@ def test_pixel_seeds_are_splitmix64():
     # SplitMix64 reference output: state 0 advanced once by the golden gamma
     s = dfa.pixel_seeds(0, 2)
     assert int(s[0]) & ((1 << 64) - 1) == 0xE220A8397B1DCDAF
+
+
+def test_batch_create_without_device_fails_loudly():
+    if dfa.device_count() > 0:
+        pytest.skip("a HIP device is visible")
+    with pytest.raises(dfa.HIPError) as e:
+        dfa.HIPBatchRenderer()
+    assert e.value.code == -2
