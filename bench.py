@@ -14,6 +14,9 @@ Besides the contract fields the JSON line carries:
                  renderer's stream), against the FP32 peak; the byte figure is
                  reported alongside;
   stages       - kernel time, launches and ray queries per step of each stage;
+  batch        - the batch mix (256 scenes of 64^2..512^2 pixels and 8..128 samples per
+                 pixel): Mrays/s rendering the scenes one after another (plain)
+                 and in one batched launch, and their ratio;
   cpu_baseline - the CPU oracle (the reference's per-pixel kernel restated in
                  C++ with the reference's thread-pool semantics) on a band of
                  rows of the same frame, on this host's cores.
@@ -138,6 +141,41 @@ def traffic_from_profiles(workload, kernel_prefix):
     return None
 
 
+def batch_measure(n_scenes):
+    """The uneven batch mix (scene seeds 100.., each at its own resolution and samples per pixel): every scene
+    rendered alone one after another (plain, the best single-scene schedule), then all in one batched launch."""
+    import dayflower_amd as dfa
+
+    scenes = [dfa.build_batch_scene(100 + i) for i in range(n_scenes)]
+    b = dfa.HIPBatchRenderer()
+    try:
+        for i, sc in enumerate(scenes):
+            b.add(sc, passes=sc.spp, seed=1 + i)
+        jobs = [b.job(i) for i in range(n_scenes)]
+        for j in jobs:
+            j.resetCounters()
+        t0 = time.perf_counter()
+        for j, sc in zip(jobs, scenes):
+            j.execute(sc.spp)
+            j.synchronize()
+        t1 = time.perf_counter()
+        rays_plain = sum(j.counters()["rays"] for j in jobs)
+        for j in jobs:
+            j.resetCounters()
+        t2 = time.perf_counter()
+        b.execute()
+        t3 = time.perf_counter()
+        rays_batched = sum(j.counters()["rays"] for j in jobs)
+    finally:
+        b.dispose()
+    plain = rays_plain / (t1 - t0) / 1e6
+    batched = rays_batched / (t3 - t2) / 1e6
+    return {"scenes": n_scenes, "samples": int(sum(sc.width * sc.height * sc.spp for sc in scenes)),
+            "plain_mrays_s": round(plain, 2), "plain_s": round(t1 - t0, 3), "batched_mrays_s": round(batched, 2),
+            "batched_s": round(t3 - t2, 3), "batched_over_plain": round(batched / plain, 3),
+            "rays_plain": rays_plain, "rays_batched": rays_batched}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +185,8 @@ def main():
     ap.add_argument("--math", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256,
+                    help="scenes of the batch mix (seeds 100, 101, ...) rendered plain and batched; 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB float copy kernel (negation), the reference for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -242,6 +282,8 @@ def main():
                          "rays_per_step": stages[n]["counters"]["rays"] / steps}
                      for n in stages if stages[n]["launches"] > 0}
 
+    batch = batch_measure(args.batch_scenes) if args.batch_scenes > 0 else None
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -272,6 +314,7 @@ def main():
         },
         "roofline": roofline,
         "stages": stage_summary,
+        "batch": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
