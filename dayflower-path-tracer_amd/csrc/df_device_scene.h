@@ -124,7 +124,30 @@ struct DParams {
   int32_t clear_first;
   // sRGB constants (image4j RGBColorSpace.SRGB)
   float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
+  // differentiable mode (diff_on = 1): continuous albedo / emission
+  // parameters replace the 8-bit textures of the surfaces mapped to them
+  int32_t diff_on;
+  int32_t n_surfaces;               // surface records (6 floats each)
+  const int32_t* surface_albedo;    // per surface record: albedo group or -1
+  const int32_t* surface_emission;  // per surface record: emission index or -1
+  const float* diff_albedo;         // groups x RGB
+  const float* diff_emission;       // white radiance per index
 };
+
+// Inputs of the adjoint (path-replay) kernel beyond those of a render.
+struct DAdjoint {
+  // device array of 4 skies: turbidity + h_t, - h_t, sun elevation + h_e, - h_e
+  const DSky* sky_off;
+  float h_t, h_e;       // the offsets (turbidity units, degrees)
+  const float* weight;  // dLoss/dfilm per pixel and channel, divided by the passes
+  double* grad;         // albedo groups x 3, emission, turbidity, elevation
+  int32_t n_albedo;
+  int32_t n_emission;
+};
+
+// Parameters of the adjoint that one launch handles: LDS holds them per block.
+constexpr int kMaxDiffParams = 1024;
+constexpr int kMaxAdjointDepth = 8;
 
 // Per-pixel frame buffers.
 struct DFrame {

@@ -28,3 +28,10 @@ hipError_t dfk_launch_seed(uint64_t* seeds, uint64_t n, uint64_t master, hipStre
 hipError_t dfk_launch_pt_wavefront(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k,
                                    const dfk::DParams& p, const dfk::DFrame& f, const dfk::DPaths& w, int math_mode,
                                    hipStream_t stream, const dfk_marks* marks = nullptr);
+
+// Differentiable path tracer (df_adjoint.hip): dLoss/dfilm weights and the
+// loss of a film against a target, then the path-replay adjoint.
+hipError_t dfk_launch_loss(const float* film, const float* target, float* weight, int n, float scale, double* loss,
+                           hipStream_t stream);
+hipError_t dfk_launch_adjoint(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k, const dfk::DParams& p,
+                              const dfk::DFrame& f, const dfk::DAdjoint& a, int math_mode, hipStream_t stream);

@@ -71,6 +71,14 @@ struct df_ctx {
   DevBuf path_f4[11], path_q[3], path_ctl;
   dfk::DPaths paths{};
 
+  // differentiable mode
+  bool diff_on = false;
+  int32_t diff_n_albedo = 0, diff_n_emission = 0;
+  float diff_h_t = 0.0f, diff_h_e = 0.0f;
+  DevBuf diff_surf_albedo, diff_surf_emission, diff_albedo, diff_emission, diff_skies, diff_weight, diff_grad,
+      diff_loss, diff_seeds;
+  size_t diff_n_surfaces = 0;
+
   // per-stage launch timing: event pairs recorded around each launch
   bool stage_timing = false;
   hipError_t mark_error = hipSuccess;
@@ -716,6 +724,17 @@ int read_back(df_ctx* c, void* dst, const DevBuf& b, size_t bytes) {
   return DF_OK;
 }
 
+// The differentiable-mode fields of the kernel parameters.
+void apply_diff(const df_ctx* c, DParams& p) {
+  p.diff_on = c->diff_on ? 1 : 0;
+  if (!c->diff_on) return;
+  p.n_surfaces = static_cast<int32_t>(c->diff_n_surfaces);
+  p.surface_albedo = static_cast<const int32_t*>(c->diff_surf_albedo.p);
+  p.surface_emission = static_cast<const int32_t*>(c->diff_surf_emission.p);
+  p.diff_albedo = static_cast<const float*>(c->diff_albedo.p);
+  p.diff_emission = static_cast<const float*>(c->diff_emission.p);
+}
+
 DFrame frame_of(df_ctx* c) {
   DFrame f{};
   f.film = static_cast<float*>(c->film.p);
@@ -853,6 +872,9 @@ void df_destroy(df_ctx* c) {
   free_buf(c->seeds);
   free_buf(c->counters);
   free_paths(c);
+  for (DevBuf* b : {&c->diff_surf_albedo, &c->diff_surf_emission, &c->diff_albedo, &c->diff_emission, &c->diff_skies,
+                    &c->diff_weight, &c->diff_grad, &c->diff_loss, &c->diff_seeds})
+    free_buf(*b);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -1052,7 +1074,8 @@ int df_execute(df_ctx* c, int32_t passes) {
   DF_HIP(c, hipSetDevice(c->device));
   const DCamera cam = make_camera(c->camera);
   const DSky sky = make_sky(c->sky);
-  const DParams par = make_params(c->params, c->width, c->height, passes, c->clear_pending);
+  DParams par = make_params(c->params, c->width, c->height, passes, c->clear_pending);
+  apply_diff(c, par);
   const DFrame f = frame_of(c);
   const bool wavefront = use_wavefront(c);
   if (wavefront && (rc = ensure_paths(c, c->params.pt_depth_maximum))) return rc;
@@ -1184,6 +1207,122 @@ const char* df_last_error(df_ctx* c) {
   return c->err.c_str();
 }
 
+// ------------------------------------------------------------ differentiable path tracer
+
+int df_set_diff(df_ctx* c, const df_diff* d) {
+  if (!c || !d) return DF_EINVAL;
+  if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
+  const size_t n_surf = c->hs.surfaces.size() / 6;
+  if (d->n_surfaces != n_surf || !d->surface_albedo || !d->surface_emission)
+    return fail(c, DF_EINVAL, "surface maps must hold one entry per surface record (%zu)", n_surf);
+  if (d->n_albedo < 0 || d->n_emission < 0 || 3 * d->n_albedo + d->n_emission + 2 > dfk::kMaxDiffParams)
+    return fail(c, DF_EINVAL, "too many parameters (%d albedo groups, %d emissions)", d->n_albedo, d->n_emission);
+  if ((d->n_albedo > 0 && !d->albedo) || (d->n_emission > 0 && !d->emission))
+    return fail(c, DF_EINVAL, "parameter arrays missing");
+  for (size_t i = 0; i < n_surf; ++i) {
+    if (d->surface_albedo[i] < -1 || d->surface_albedo[i] >= d->n_albedo)
+      return fail(c, DF_EINVAL, "surface %zu: albedo group %d", i, d->surface_albedo[i]);
+    if (d->surface_emission[i] < -1 || d->surface_emission[i] >= d->n_emission)
+      return fail(c, DF_EINVAL, "surface %zu: emission index %d", i, d->surface_emission[i]);
+  }
+  if (!(d->h_turbidity > 0.0f) || !(d->h_elevation > 0.0f)) return fail(c, DF_EINVAL, "parameter offsets must be > 0");
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  const std::vector<int32_t> sa(d->surface_albedo, d->surface_albedo + n_surf);
+  const std::vector<int32_t> se(d->surface_emission, d->surface_emission + n_surf);
+  const std::vector<float> al(d->albedo, d->albedo + 3 * d->n_albedo);
+  const std::vector<float> em(d->emission, d->emission + d->n_emission);
+  const std::vector<DSky> skies = {make_sky(d->sky_turbidity[0]), make_sky(d->sky_turbidity[1]),
+                                   make_sky(d->sky_elevation[0]), make_sky(d->sky_elevation[1])};
+  int rc;
+  if ((rc = upload(c, c->diff_surf_albedo, sa)) || (rc = upload(c, c->diff_surf_emission, se)) ||
+      (rc = upload(c, c->diff_albedo, al)) || (rc = upload(c, c->diff_emission, em)) ||
+      (rc = upload(c, c->diff_skies, skies)))
+    return rc;
+  c->diff_n_surfaces = n_surf;
+  c->diff_n_albedo = d->n_albedo;
+  c->diff_n_emission = d->n_emission;
+  c->diff_h_t = d->h_turbidity;
+  c->diff_h_e = d->h_elevation;
+  c->diff_on = true;
+  return DF_OK;
+}
+
+int df_clear_diff(df_ctx* c) {
+  if (!c) return DF_EINVAL;
+  c->diff_on = false;
+  return DF_OK;
+}
+
+int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, double* loss, double* grad,
+                     size_t n_grad, float* forward_ms, float* adjoint_ms) {
+  int rc = need_frame(c);
+  if (rc) return rc;
+  if (!c->diff_on) return fail(c, DF_ESTATE, "df_set_diff has not been called");
+  if (!target || n != pixels(c) * 3) return fail(c, DF_EINVAL, "target length %zu != %zu", n, pixels(c) * 3);
+  if (passes < 1) return fail(c, DF_EINVAL, "passes %d", passes);
+  const df_params& p = c->params;
+  if (p.renderer_type != DF_RENDERER_PATH_TRACER) return fail(c, DF_EINVAL, "the adjoint needs the path tracer");
+  if (grad) {
+    const size_t need = static_cast<size_t>(3 * c->diff_n_albedo + c->diff_n_emission + 2);
+    if (n_grad != need) return fail(c, DF_EINVAL, "gradient length %zu != %zu", n_grad, need);
+    if (p.pt_depth_maximum > dfk::kMaxAdjointDepth)
+      return fail(c, DF_EINVAL, "the adjoint supports pt_depth_maximum <= %d", dfk::kMaxAdjointDepth);
+    if (p.pt_depth_russian_roulette <= p.pt_depth_maximum)
+      return fail(c, DF_EINVAL, "the adjoint needs Russian roulette off (pt_depth_russian_roulette > pt_depth_maximum)");
+  }
+  DF_HIP(c, hipSetDevice(c->device));
+  const size_t np = pixels(c);
+  if ((rc = alloc_buf(c, c->diff_weight, np * 3 * sizeof(float))) || (rc = alloc_buf(c, c->diff_loss, sizeof(double))) ||
+      (rc = alloc_buf(c, c->diff_seeds, np * sizeof(uint64_t))))
+    return rc;
+  // the forward pass starts from these seeds; the replay starts from them again
+  DF_HIP(c, hipMemcpyAsync(c->diff_seeds.p, c->seeds.p, np * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+  c->clear_pending = true;
+  if ((rc = df_execute(c, passes))) return rc;
+  float fms = 0.0f;
+  DF_HIP(c, hipEventSynchronize(c->ev1));
+  DF_HIP(c, hipEventElapsedTime(&fms, c->ev0, c->ev1));
+  if (forward_ms) *forward_ms = fms;
+  DF_HIP(c, hipMemcpy(c->diff_weight.p, target, n * sizeof(float), hipMemcpyHostToDevice));
+  DF_HIP(c, hipMemsetAsync(c->diff_loss.p, 0, sizeof(double), c->stream));
+  // weights in place: dLoss/dfilm / passes
+  DF_HIP(c, dfk_launch_loss(static_cast<const float*>(c->film.p), static_cast<const float*>(c->diff_weight.p),
+                            static_cast<float*>(c->diff_weight.p), static_cast<int>(n), 1.0f / static_cast<float>(passes),
+                            static_cast<double*>(c->diff_loss.p), c->stream));
+  if (grad) {
+    const size_t ng = n_grad;
+    if ((rc = alloc_buf(c, c->diff_grad, ng * sizeof(double)))) return rc;
+    DF_HIP(c, hipMemsetAsync(c->diff_grad.p, 0, ng * sizeof(double), c->stream));
+    // replay from the forward's starting seeds; the replay leaves the seeds as the forward did
+    DF_HIP(c, hipMemcpyAsync(c->seeds.p, c->diff_seeds.p, np * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
+    dfk::DAdjoint a{};
+    a.sky_off = static_cast<const DSky*>(c->diff_skies.p);
+    a.h_t = c->diff_h_t;
+    a.h_e = c->diff_h_e;
+    a.weight = static_cast<const float*>(c->diff_weight.p);
+    a.grad = static_cast<double*>(c->diff_grad.p);
+    a.n_albedo = c->diff_n_albedo;
+    a.n_emission = c->diff_n_emission;
+    const DCamera cam = make_camera(c->camera);
+    const DSky sky = make_sky(c->sky);
+    DParams par = make_params(c->params, c->width, c->height, passes, false);
+    apply_diff(c, par);
+    DF_HIP(c, hipEventRecord(c->ev0, c->stream));
+    DF_HIP(c, dfk_launch_adjoint(c->ds, cam, sky, par, frame_of(c), a, p.math_mode, c->stream));
+    DF_HIP(c, hipEventRecord(c->ev1, c->stream));
+    DF_HIP(c, hipEventSynchronize(c->ev1));
+    float ams = 0.0f;
+    DF_HIP(c, hipEventElapsedTime(&ams, c->ev0, c->ev1));
+    if (adjoint_ms) *adjoint_ms = ams;
+    DF_HIP(c, hipMemcpy(grad, c->diff_grad.p, ng * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  double l = 0.0;
+  DF_HIP(c, hipMemcpy(&l, c->diff_loss.p, sizeof(double), hipMemcpyDeviceToHost));
+  if (loss) *loss = l;
+  return DF_OK;
+}
+
 // ------------------------------------------------------------ batches of scenes
 
 int df_batch_create(df_batch** out, int32_t device) {
@@ -1271,6 +1410,7 @@ int df_batch_execute(df_batch* b) {
     hj[j].C = make_camera(c->camera);
     hj[j].K = make_sky(c->sky);
     hj[j].P = make_params(c->params, c->width, c->height, b->passes[j], c->clear_pending);
+    apply_diff(c, hj[j].P);
     hj[j].F = frame_of(c);
     if (b->passes[j] == 0) continue;
     const int key = c->params.math_mode * 8 + c->params.renderer_type;

@@ -12,6 +12,23 @@
 
 namespace dfk {
 
+// What one path-tracer bounce contributed, for the adjoint's replay: the
+// albedo / emission parameters it used, the sun term's albedo factor and its
+// elevation derivative, and how the throughput changed (m *= s, times the
+// albedo when uses_albedo).  Inputs: the offset skies for the derivative.
+struct Vtx {
+  const DSky* sky_e[2];  // in: sun elevation + h_e, - h_e
+  float h_e;             // in
+  bool miss;
+  bool uses_albedo;
+  bool rr;               // Russian roulette scaled the albedo
+  int aslot, eslot;
+  float a[3];
+  float e[3];            // emission added at the vertex
+  float s;
+  float sun_w, dsun_w;
+};
+
 template <int M>
 struct Tracer {
   const DScene& S;
@@ -666,45 +683,62 @@ struct Tracer {
   // Samples a direction towards the sun.  Returns true when a shadow query
   // towards `sd` decides the result; tmp then holds the contribution when
   // nothing is hit (and 0 otherwise, as it also does on false).
+  // A direction in the cone towards the sun of sky k for the draws (u, v);
+  // returns dp0 = d . sun direction (:2250-2329).
+  __device__ __forceinline__ float sun_cone_dir(const DSky& k, float pX, float pY, float pZ, float u, float v,
+                                                float d[3]) {
+    const float sdX = k.sun_dir_world[0], sdY = k.sun_dir_world[1], sdZ = k.sun_dir_world[2];
+    const float sunR2 = 500.0f * 500.0f;
+    const float toX = k.sun_origin[0] - pX, toY = k.sun_origin[1] - pY, toZ = k.sun_origin[2] - pZ;
+    const float to2 = toX * toX + toY * toY + toZ * toZ;
+    const float sinMax2 = sunR2 / to2;
+    const float cosMax = sqrtf(jmax(0.0f, 1.0f - sinMax2));
+    const float cosT = u * (cosMax - 1.0f) + 1.0f;
+    const float sinT = sqrtf(jmax(0.0f, 1.0f - cosT * cosT));
+    const float phi = DF_PI_2 * v;
+    const float x = Math<M>::cos(phi) * sinT, y = Math<M>::sin(phi) * sinT, z = cosT;
+    const float wX = sdX, wY = sdY, wZ = sdZ;
+    const float aX = fabsf(wX), aY = fabsf(wY), aZ = fabsf(wZ);
+    const bool c0 = aX < aY && aX < aZ;
+    const bool c1 = aY < aZ;
+    const float vX = c0 ? 0.0f : c1 ? wZ : wY;
+    const float vY = c0 ? wZ : c1 ? 0.0f : -wX;
+    const float vZ = c0 ? -wY : c1 ? -wX : 0.0f;
+    const float vr = jrsqrt(vX * vX + vY * vY + vZ * vZ);
+    const float vnX = vX * vr, vnY = vY * vr, vnZ = vZ * vr;
+    const float unX = vnY * wZ - vnZ * wY, unY = vnZ * wX - vnX * wZ, unZ = vnX * wY - vnY * wX;
+    const float rX = unX * x + vnX * y + wX * z;
+    const float rY = unY * x + vnY * y + wY * z;
+    const float rZ = unZ * x + vnZ * y + wZ * z;
+    const float rr = jrsqrt(rX * rX + rY * rY + rZ * rZ);
+    d[0] = rX * rr;
+    d[1] = rY * rr;
+    d[2] = rZ * rr;
+    return d[0] * sdX + d[1] * sdY + d[2] * sdZ;
+  }
+
+  // Samples a direction towards the sun (two draws, returned in uv).
+  // Returns true when a shadow query towards `sd` decides the result; tmp
+  // then holds the contribution when nothing is hit (and 0 otherwise, as it
+  // also does on false).
   __device__ __forceinline__ bool sun_sample(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ,
-                                             float aR, float aG, float aB, float sd[3]) {  // :2238-2347
+                                             float aR, float aG, float aB, float sd[3],
+                                             float uv[2]) {  // :2238-2347
     bool query_needed = false;
     float r = 0.0f, g = 0.0f, b = 0.0f;
     if (active) {
-      const float sdX = K.sun_dir_world[0], sdY = K.sun_dir_world[1], sdZ = K.sun_dir_world[2];
-      const float sunR2 = 500.0f * 500.0f;
-      const float toX = K.sun_origin[0] - pX, toY = K.sun_origin[1] - pY, toZ = K.sun_origin[2] - pZ;
-      const float to2 = toX * toX + toY * toY + toZ * toZ;
       const float u = rng.next();
       const float v = rng.next();
-      const float sinMax2 = sunR2 / to2;
-      const float cosMax = sqrtf(jmax(0.0f, 1.0f - sinMax2));
-      const float cosT = u * (cosMax - 1.0f) + 1.0f;
-      const float sinT = sqrtf(jmax(0.0f, 1.0f - cosT * cosT));
-      const float phi = DF_PI_2 * v;
-      const float x = Math<M>::cos(phi) * sinT, y = Math<M>::sin(phi) * sinT, z = cosT;
-      const float wX = sdX, wY = sdY, wZ = sdZ;
-      const float aX = fabsf(wX), aY = fabsf(wY), aZ = fabsf(wZ);
-      const bool c0 = aX < aY && aX < aZ;
-      const bool c1 = aY < aZ;
-      const float vX = c0 ? 0.0f : c1 ? wZ : wY;
-      const float vY = c0 ? wZ : c1 ? 0.0f : -wX;
-      const float vZ = c0 ? -wY : c1 ? -wX : 0.0f;
-      const float vr = jrsqrt(vX * vX + vY * vY + vZ * vZ);
-      const float vnX = vX * vr, vnY = vY * vr, vnZ = vZ * vr;
-      const float unX = vnY * wZ - vnZ * wY, unY = vnZ * wX - vnX * wZ, unZ = vnX * wY - vnY * wX;
-      const float rX = unX * x + vnX * y + wX * z;
-      const float rY = unY * x + vnY * y + wY * z;
-      const float rZ = unZ * x + vnZ * y + wZ * z;
-      const float rr = jrsqrt(rX * rX + rY * rY + rZ * rZ);
-      const float dX = rX * rr, dY = rY * rr, dZ = rZ * rr;
-      const float dp0 = dX * sdX + dY * sdY + dZ * sdZ;
-      const float dp1 = dX * nX + dY * nY + dZ * nZ;
+      uv[0] = u;
+      uv[1] = v;
+      float d[3];
+      const float dp0 = sun_cone_dir(K, pX, pY, pZ, u, v, d);
+      const float dp1 = d[0] * nX + d[1] * nY + d[2] * nZ;
       if (dp0 > 0.0f && dp1 > 0.0f) {
         query_needed = true;
-        sd[0] = dX;
-        sd[1] = dY;
-        sd[2] = dZ;
+        sd[0] = d[0];
+        sd[1] = d[1];
+        sd[2] = d[2];
         r = aR * 1.0f * dp1 * DF_PI_RECIP;
         g = aG * 1.0f * dp1 * DF_PI_RECIP;
         b = aB * 1.0f * dp1 * DF_PI_RECIP;
@@ -714,6 +748,16 @@ struct Tracer {
     tmp[1] = g;
     tmp[2] = b;
     return query_needed;
+  }
+
+  // dp1 / pi of the sun sample of sky k at the draws uv (0 where the sample
+  // is rejected): the factor of the albedo in the sun term.
+  __device__ __forceinline__ float sun_weight(const DSky& k, float pX, float pY, float pZ, float nX, float nY,
+                                              float nZ, const float uv[2]) {
+    float d[3];
+    const float dp0 = sun_cone_dir(k, pX, pY, pZ, uv[0], uv[1], d);
+    const float dp1 = d[0] * nX + d[1] * nY + d[2] * nZ;
+    return dp0 > 0.0f && dp1 > 0.0f ? dp1 * DF_PI_RECIP : 0.0f;
   }
 
   // The shadow query's verdict: nothing hit (:2331).
@@ -765,7 +809,7 @@ struct Tracer {
   // query).  depth counts the bounces taken.
   __device__ __forceinline__ bool pt_bounce(float o[3], float d[3], float m[3], float p[3], int& depth,
                                             bool sky_active, bool sun_active, bool& shadow, float sd[3],
-                                            float pv[3]) {
+                                            float pv[3], Vtx* vtx = nullptr) {
     const int depth_rr = P.depth_rr;
     float ox = o[0], oy = o[1], oz = o[2], dx = d[0], dy = d[1], dz = d[2];
     float pr = p[0], pg = p[1], pb = p[2];
@@ -773,6 +817,7 @@ struct Tracer {
     shadow = false;
     const int po = h.prim;
     if (h.t == DF_INF || po == -1) {
+      if (vtx) vtx->miss = true;
       sky(sky_active, dx, dy, dz);
       pr += mr * tmp[0] * DF_PI_RECIP;
       pg += mg * tmp[1] * DF_PI_RECIP;
@@ -790,15 +835,62 @@ struct Tracer {
     const float dp2 = dp * 2.0f;
     const bool ok = dp < 0.0f;
     const float wX = ok ? sX : -sX, wY = ok ? sY : -sY, wZ = ok ? sZ : -sZ;
-    const int alb = tex_color(jf2i(S.surfaces[so + 1]));
-    float aR = ch_r(alb), aG = ch_g(alb), aB = ch_b(alb);
-    const int emi = tex_color(jf2i(S.surfaces[so + 2]));
-    const float eR = ch_r(emi), eG = ch_g(emi), eB = ch_b(emi);
+    float aR, aG, aB, eR, eG, eB;
+    int aslot = -1, eslot = -1;
+    if (P.diff_on) {  // differentiable mode: parameters instead of 8-bit textures
+      const int rec_i = so / 6;
+      if (rec_i >= 0 && rec_i < P.n_surfaces) {
+        aslot = P.surface_albedo[rec_i];
+        eslot = P.surface_emission[rec_i];
+      }
+    }
+    if (aslot >= 0) {
+      aR = P.diff_albedo[aslot * 3 + 0];
+      aG = P.diff_albedo[aslot * 3 + 1];
+      aB = P.diff_albedo[aslot * 3 + 2];
+    } else {
+      const int alb = tex_color(jf2i(S.surfaces[so + 1]));
+      aR = ch_r(alb);
+      aG = ch_g(alb);
+      aB = ch_b(alb);
+    }
+    if (eslot >= 0) {
+      eR = eG = eB = P.diff_emission[eslot];
+    } else {
+      const int emi = tex_color(jf2i(S.surfaces[so + 2]));
+      eR = ch_r(emi);
+      eG = ch_g(emi);
+      eB = ch_b(emi);
+    }
     const int mat = jf2i(S.surfaces[so]);
+    float uv[2] = {0.0f, 0.0f};
     if (mat == 1 || mat == 3 || mat == 4) {
-      shadow = sun_sample(sun_active, pX, pY, pZ, wX, wY, wZ, aR, aG, aB, sd);
+      shadow = sun_sample(sun_active, pX, pY, pZ, wX, wY, wZ, aR, aG, aB, sd, uv);
     } else {
       tmp[0] = tmp[1] = tmp[2] = 0.0f;
+    }
+    if (vtx) {
+      vtx->miss = false;
+      vtx->aslot = aslot;
+      vtx->eslot = eslot;
+      vtx->a[0] = aR;
+      vtx->a[1] = aG;
+      vtx->a[2] = aB;
+      vtx->e[0] = eR;
+      vtx->e[1] = eG;
+      vtx->e[2] = eB;
+      vtx->sun_w = 0.0f;
+      vtx->dsun_w = 0.0f;
+      if (shadow) {
+        // the sun term is a * dp1 / pi; its change with the sun elevation at
+        // the same (u, v), by central differences over the offset skies
+        vtx->sun_w = sun_weight(K, pX, pY, pZ, wX, wY, wZ, uv);
+        vtx->dsun_w = (sun_weight(*vtx->sky_e[0], pX, pY, pZ, wX, wY, wZ, uv) -
+                       sun_weight(*vtx->sky_e[1], pX, pY, pZ, wX, wY, wZ, uv)) / (2.0f * vtx->h_e);
+      }
+      vtx->s = 1.0f;
+      vtx->uses_albedo = false;
+      vtx->rr = false;
     }
     if (shadow) {
       pv[0] = pr + mr * (eR + tmp[0]);
@@ -824,6 +916,7 @@ struct Tracer {
         return false;
       }
       const float pdfr = 1.0f / pdf;
+      if (vtx) vtx->rr = true;
       aR *= pdfr;
       aG *= pdfr;
       aB *= pdfr;
@@ -853,6 +946,7 @@ struct Tracer {
           const bool isR = rng.next() < prob;
           const float m = isR ? rp : tp;
           mr *= m;
+          if (vtx) vtx->s = m;
           mg *= m;
           mb *= m;
           if (isR) {
@@ -872,6 +966,7 @@ struct Tracer {
             dy = d[1];
             dz = d[2];
             mr *= aR;
+            if (vtx) vtx->uses_albedo = true;
             mg *= aG;
             mb *= aB;
           }
@@ -886,6 +981,7 @@ struct Tracer {
         dy = d[1];
         dz = d[2];
         mr *= aR;
+        if (vtx) vtx->uses_albedo = true;
         mg *= aG;
         mb *= aB;
       } else if (mat == 4) {  // Phong :3516-3576
@@ -916,6 +1012,7 @@ struct Tracer {
         dy = qY * qr;
         dz = qZ * qr;
         mr *= aR;
+        if (vtx) vtx->uses_albedo = true;
         mg *= aG;
         mb *= aB;
       } else if (mat == 2) {  // Glass :3577-3670
@@ -949,6 +1046,7 @@ struct Tracer {
           const bool isR = rng.next() < prob;
           const float m = isR ? rp : tp;
           mr *= m;
+          if (vtx) vtx->s = m;
           mg *= m;
           mb *= m;
           const float eps = isR ? 0.01f : 0.000001f;
@@ -967,6 +1065,7 @@ struct Tracer {
         dy = dy - sY * dp2;
         dz = dz - sZ * dp2;
         mr *= aR;
+        if (vtx) vtx->uses_albedo = true;
         mg *= aG;
         mb *= aB;
       }

@@ -456,6 +456,78 @@ Compiled build_batch_scene(uint64_t seed) {
   return c;
 }
 
+// The differentiation scene: 512x512, 32 spp, depth 6 (Russian roulette off:
+// depth 7).  A noisy icosphere subdivided one level further than the mesh
+// scene (20 * 4^7 = 327,680 triangles, fBm amplitude 0.15, 5 octaves, seed 7)
+// in 48 groups of consecutive triangles, plus 20,000 scattered triangles
+// (seed 8) in 16 groups by x/z quadrant: 64 TriangleMesh primitives, each
+// with its own surface (albedo group k = the k-th mesh primitive), materials
+// uniform over {Lambert, Phong, ClearCoat, Glass}.  Four emissive area-light
+// triangles (emission j = the j-th triangle primitive) above the scene, a
+// checkerboard ground.  The albedo, emission, turbidity and sun elevation
+// that a render uses come from the differentiable-mode parameters; the
+// textures here are the initial values.
+Compiled config5(uint64_t seed) {
+  Rng mat_rng(seed ? seed : 9);
+  Scene s;
+  s.name = "config5_diff_mesh_512";
+  const int checker = checkerboard_texture(s, gray(0.8f), gray(0.2f), 0.0f, 0.5f, 0.5f);
+  add_ground(s, kMaterialLambertian, checker);
+  static const int kMats[4] = {kMaterialLambertian, kMaterialPhong, kMaterialClearCoat, kMaterialGlass};
+  const int black = constant_texture(s, gray(0.0f));
+  std::vector<Triangle> ico = noisy_icosphere(7, 2.5f, Vec3{0.0f, 3.2f, 0.0f}, 0.15, 5, 7);
+  const size_t per = ico.size() / 48;
+  for (int g = 0; g < 48; ++g) {
+    std::vector<Triangle> part(ico.begin() + g * per, g == 47 ? ico.end() : ico.begin() + (g + 1) * per);
+    const int alb = constant_texture(s, random_color(mat_rng, 0.2f, 0.9f));
+    add_primitive(s, make_mesh(std::move(part)), add_surface(s, kMats[mat_rng.irange(0, 3)], alb, black, black));
+  }
+  Rng sr(8);
+  std::vector<Triangle> sc = scattered_triangles(20000, Vec3{-5, 0, -5}, Vec3{5, 10, 5}, 0.01f, 0.2f, sr);
+  std::vector<Triangle> groups[16];
+  for (const Triangle& t : sc) {
+    const float cx = (t.a.position.x + t.b.position.x + t.c.position.x) / 3.0f;
+    const float cz = (t.a.position.z + t.b.position.z + t.c.position.z) / 3.0f;
+    const int gx = std::min(3, std::max(0, static_cast<int>((cx + 5.0f) / 2.5f)));
+    const int gz = std::min(3, std::max(0, static_cast<int>((cz + 5.0f) / 2.5f)));
+    groups[gx * 4 + gz].push_back(t);
+  }
+  for (int g = 0; g < 16; ++g) {
+    const int alb = constant_texture(s, random_color(mat_rng, 0.2f, 0.9f));
+    const int mat = kMats[mat_rng.irange(0, 3)];
+    if (groups[g].empty()) continue;
+    add_primitive(s, make_mesh(std::move(groups[g])), add_surface(s, mat, alb, black, black));
+  }
+  for (int j = 0; j < 4; ++j) {  // area lights around and above the sphere, facing down
+    const float a = static_cast<float>(j) * 1.5707964f + 0.4f;
+    const float cx = 6.0f * std::cos(a), cz = 6.0f * std::sin(a);
+    Triangle t;
+    Vertex* vs[3] = {&t.a, &t.b, &t.c};
+    const Vec3 p[3] = {Vec3{cx - 1.0f, 9.0f, cz - 1.0f}, Vec3{cx + 1.0f, 9.0f, cz - 1.0f}, Vec3{cx, 9.0f, cz + 1.0f}};
+    for (int k = 0; k < 3; ++k) {
+      vs[k]->position = p[k];
+      vs[k]->normal = Vec3{0.0f, -1.0f, 0.0f};
+      vs[k]->uv = Vec2{k == 1 ? 1.0f : 0.0f, k == 2 ? 1.0f : 0.0f};
+      vs[k]->tangent = Vec3{1, 0, 0};
+    }
+    // distinct albedo textures keep the four surfaces distinct records
+    const int alb = constant_texture(s, gray(0.80f + 0.01f * static_cast<float>(j)));
+    const int emi = constant_texture(s, gray(1.0f));
+    add_primitive(s, make_triangle(t), add_surface(s, kMaterialLambertian, alb, emi, black));
+  }
+  finish_camera(s, 512, 512, 40.0f, Vec3{0.0f, 5.0f, 17.0f}, Vec3{0.0f, 3.0f, 0.0f}, 0.0f, 17.0f);
+  sky_set(&s.sky, sun_direction(45.0f, 30.0f), 3.0f);
+  Compiled c = compile(s);
+  c.width = 512;
+  c.height = 512;
+  c.spp = 32;
+  c.params = default_params();
+  c.params.pt_depth_maximum = 6;
+  c.params.pt_depth_russian_roulette = 7;
+  c.params.tone_mapper_type = DF_TONE_MAPPER_FILMIC_CURVE_ACES_MODIFIED;
+  return c;
+}
+
 Compiled build_config(int config_id, uint64_t seed) {
   switch (config_id) {
     case 1:
@@ -464,6 +536,8 @@ Compiled build_config(int config_id, uint64_t seed) {
       return config2(seed);
     case 3:
       return config3(seed);
+    case 5:
+      return config5(seed);
     default:
       return Compiled{};
   }

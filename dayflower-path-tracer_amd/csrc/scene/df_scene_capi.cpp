@@ -32,7 +32,7 @@ int wrap(dfs::Compiled&& c, dfs_scene** out) {
 extern "C" {
 
 int dfs_build_config(int config_id, uint64_t seed, dfs_scene** out) {
-  if (!out || config_id < 1 || config_id > 3) return DF_EINVAL;
+  if (!out || !(config_id >= 1 && config_id <= 3) && config_id != 5) return DF_EINVAL;
   try {
     return wrap(dfs::build_config(config_id, seed), out);
   } catch (const std::bad_alloc&) {

@@ -22,7 +22,10 @@ from .scene import (  # noqa: F401
     build_config,
     camera_setup,
     default_params,
+    diff_maps,
     pixel_seeds,
     read_scene,
+    sky_at,
     sky_set,
+    sun_direction,
 )

@@ -72,6 +72,22 @@ class Sky(ctypes.Structure):
     ]
 
 
+class Diff(ctypes.Structure):
+    _fields_ = [
+        ("n_albedo", ctypes.c_int32),
+        ("n_emission", ctypes.c_int32),
+        ("surface_albedo", ctypes.POINTER(ctypes.c_int32)),
+        ("surface_emission", ctypes.POINTER(ctypes.c_int32)),
+        ("n_surfaces", ctypes.c_size_t),
+        ("albedo", ctypes.POINTER(ctypes.c_float)),
+        ("emission", ctypes.POINTER(ctypes.c_float)),
+        ("sky_turbidity", Sky * 2),
+        ("sky_elevation", Sky * 2),
+        ("h_turbidity", ctypes.c_float),
+        ("h_elevation", ctypes.c_float),
+    ]
+
+
 class Params(ctypes.Structure):
     _fields_ = [
         ("renderer_type", ctypes.c_int32),
@@ -146,6 +162,11 @@ HIP_PROTOTYPES = {
     "df_get_stage_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(Counters),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "df_last_error": (ctypes.c_char_p, [_P]),
+    "df_set_diff": (ctypes.c_int, [_P, ctypes.POINTER(Diff)]),
+    "df_clear_diff": (ctypes.c_int, [_P]),
+    "df_loss_gradient": (ctypes.c_int, [_P, c_float_p, ctypes.c_size_t, ctypes.c_int32,
+                                        ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.c_size_t, c_float_p, c_float_p]),
     "df_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32]),
     "df_batch_destroy": (None, [_P]),
     "df_batch_add": (ctypes.c_int, [_P, ctypes.POINTER(CompiledScene), ctypes.POINTER(Sky), ctypes.POINTER(Params),

@@ -37,6 +37,35 @@ def sky_set(direction, turbidity=2.0):
     return s
 
 
+def sun_direction(elevation_deg, azimuth_deg):
+    """World direction towards the sun: y up, azimuth measured from +z towards +x."""
+    e, a = np.radians(float(elevation_deg)), np.radians(float(azimuth_deg))
+    return (np.cos(e) * np.sin(a), np.sin(e), np.cos(e) * np.cos(a))
+
+
+def sky_at(elevation_deg, azimuth_deg, turbidity):
+    """Sky.set for a sun at (elevation, azimuth) and a turbidity."""
+    return sky_set(sun_direction(elevation_deg, azimuth_deg), turbidity)
+
+
+def diff_maps(scene):
+    """Differentiable-mode parameter maps of a scene: albedo group k is the k-th TriangleMesh primitive's
+    surface, emission j the j-th Triangle primitive's.  Returns (surface_albedo, surface_emission, n_albedo,
+    n_emission), the maps holding one entry per surface record (-1: the textures)."""
+    n_surf = scene.surfaces.size // 6
+    sa = np.full(n_surf, -1, np.int32)
+    se = np.full(n_surf, -1, np.int32)
+    na = ne = 0
+    for st, _, so in scene.primitives.reshape(-1, 3):
+        if st == 5 and sa[so // 6] < 0:
+            sa[so // 6] = na
+            na += 1
+        elif st == 4 and se[so // 6] < 0:
+            se[so // 6] = ne
+            ne += 1
+    return sa, se, na, ne
+
+
 def camera_setup(width, height, fov_x, eye, target, aperture=0.0, focal=30.0):
     """Camera array for a look-at (Camera.java:743-748, 877-946)."""
     cam = np.zeros(19, np.float32)

@@ -139,6 +139,23 @@ int df_get_stage_stats(df_ctx* ctx, int32_t stage, df_counters* counters, double
 /* Message of the last failed call on this handle (never NULL). */
 const char* df_last_error(df_ctx* ctx);
 
+/* ---- Differentiable path tracer -------------------------------------------
+ * (no counterpart in the reference; planned with the ABI as df_adjoint).
+ * df_set_diff switches the path tracer to the parameters of *diff (copied);
+ * df_clear_diff restores the textures.  df_loss_gradient renders `passes`
+ * samples per pixel into a cleared film (the forward pass), returns
+ * L = sum (film - target)^2 over the W x H x 3 linear film, and, when grad
+ * is not NULL, dL/dparameters (n_grad = 3 n_albedo + n_emission + 2) by
+ * replaying every path from the seeds the forward pass started from.  It
+ * needs the path tracer, Russian roulette off (pt_depth_russian_roulette >
+ * pt_depth_maximum) and pt_depth_maximum <= 8.  forward_ms / adjoint_ms
+ * (may be NULL) receive the device times of the two passes.  On return the
+ * seeds are those after the forward pass. */
+int df_set_diff(df_ctx* ctx, const df_diff* diff);
+int df_clear_diff(df_ctx* ctx);
+int df_loss_gradient(df_ctx* ctx, const float* target, size_t n, int32_t passes, double* loss, double* grad,
+                     size_t n_grad, float* forward_ms, float* adjoint_ms);
+
 /* ---- Batches of independent scenes ---------------------------------------
  * The reference renders one scene per kernel; these entry points (planned
  * with the ABI as df_batch_*) render many scenes in one persistent launch:

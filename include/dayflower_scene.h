@@ -37,8 +37,9 @@ typedef struct dfs_stats {
 
 /* Builds benchmark workload `config_id` (1: 256x256 spheres + checkerboard
  * plane; 2: 1024x768 ~100k-triangle mesh; 3: 1920x1080 mesh + terrain + 50
- * spheres + emissive triangle).  seed 0 selects the workload's documented
- * seed.  Replaces SceneLoader.loadCompiledScene (SceneLoader.java:113-133). */
+ * spheres + emissive triangle; 5: 512x512 differentiation scene, 347,680
+ * triangles in 64 mesh groups + 4 area lights).  seed 0 selects the
+ * workload's documented seed.  Replaces SceneLoader.loadCompiledScene (SceneLoader.java:113-133). */
 int dfs_build_config(int config_id, uint64_t seed, dfs_scene** out);
 
 /* Builds one scene of the uneven batch mix from its seed (100..355). */

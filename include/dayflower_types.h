@@ -144,6 +144,28 @@ typedef struct df_params {
   int32_t pt_engine;        /* DF_PT_* */
 } df_params;
 
+/* Differentiable mode of the path tracer (no counterpart in the reference).
+ * Continuous parameters replace the 8-bit albedo / emission textures of the
+ * surfaces mapped to them; the sky's turbidity and the sun's elevation are
+ * differentiated through skies set up at parameter offsets (Sky.set of the
+ * current sky with turbidity +- h_turbidity, and with the sun elevation +-
+ * h_elevation degrees at the same azimuth).  Gradient layout: albedo groups
+ * (n_albedo x RGB), emission (n_emission), turbidity, sun elevation (per
+ * degree). */
+typedef struct df_diff {
+  int32_t n_albedo;
+  int32_t n_emission;
+  const int32_t* surface_albedo;    /* per surface record (surfaces_len / 6): albedo group or -1 */
+  const int32_t* surface_emission;  /* per surface record: emission index or -1 */
+  size_t n_surfaces;
+  const float* albedo;              /* n_albedo * 3, linear RGB */
+  const float* emission;            /* n_emission, white radiance */
+  df_sky sky_turbidity[2];          /* the sky at turbidity + h, - h */
+  df_sky sky_elevation[2];          /* the sky at sun elevation + h, - h */
+  float h_turbidity;
+  float h_elevation;
+} df_diff;
+
 /* Work counters.  A ray is one intersection query (closest-hit or any-hit);
  * a sample is one film update of one pixel.  The remaining fields count the
  * algorithmic units of the intersection and sky work: BVH box tests,
