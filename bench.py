@@ -47,11 +47,12 @@ FLOP_PER = {"box_tests": 24, "tri_tests": 45, "prim_tests": 36, "sphere_tests": 
             "sky_evals": 60}
 BYTES_PER = {"box_tests": 32, "tri_tests": 36, "prim_tests": 48, "sphere_tests": 16}
 PEAK_FP32_TFLOPS = 157.3
-KERNEL_SYMBOLS = {"megakernel": "dfk::render_kernel", "raygen": "dfk::wf_raygen", "closest": "dfk::wf_closest",
-                  "shade": "dfk::wf_shade", "shadow": "dfk::wf_shadow", "escape": "dfk::wf_escape", "film": "dfk::wf_film"}
+KERNEL_SYMBOLS = {"megakernel": "dfk::render_kernel<0, 2>", "raygen": "dfk::wf_raygen<0>",
+                  "trace": "dfk::wf_trace<0, false>", "shade": "dfk::wf_shade<0>", "escape": "dfk::wf_trace<0, true>",
+                  "film": "dfk::wf_film<0>"}
 KERNEL_NAMES = {"megakernel": "render_kernel (one work-item per pixel)", "raygen": "wf_raygen",
-                "closest": "wf_closest (closest-hit queries)", "shade": "wf_shade", "shadow": "wf_shadow (sun any-hit)",
-                "escape": "wf_escape", "film": "wf_film"}
+                "trace": "wf_trace<0,false> (a bounce's sun any-hit + closest-hit queries)", "shade": "wf_shade",
+                "escape": "wf_trace<0,true> (escape tests)", "film": "wf_film"}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -129,16 +130,14 @@ def traffic_summary(out_path, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv
     print(json.dumps(summary))
 
 
-def traffic_from_profiles(workload, kernel_prefix):
+def traffic_from_profiles(workload, kernel_symbol):
     """HBM bytes per dispatch of the kernel from profiles/traffic_workload<N>.json, or None."""
     path = os.path.join(REPO, "profiles", "traffic_workload%d.json" % workload)
     if not os.path.exists(path):
         return None
     data = json.load(open(path))
-    for name, k in data.get("kernels", {}).items():
-        if name.split("<")[0] == kernel_prefix:
-            return k["hbm_bytes_per_dispatch"]
-    return None
+    k = data.get("kernels", {}).get(kernel_symbol)
+    return k["hbm_bytes_per_dispatch"] if k else None
 
 
 def batch_measure(n_scenes):
@@ -232,6 +231,10 @@ def main():
     k.update(scene.width, scene.height)
     k.updateCamera()
     k.seed(seed)
+    tune_passes = 0
+    while k.engine() == dfa.PT_AUTO:  # the autotuner times both schedules on the first passes
+        k.execute(1)
+        tune_passes += 1
     for _ in range(args.warmup):
         k.execute(1)
     k.synchronize()
@@ -248,6 +251,7 @@ def main():
     t1 = time.perf_counter()
     c = k.counters()
     stages = k.stageStats()
+    k_engine = k.engine()
     k.dispose()
 
     wall = t1 - t0
@@ -309,6 +313,8 @@ def main():
                         "%s math" % (scene.name, scene.width, scene.height, params.pt_depth_maximum,
                                      stats["triangles"], stats["spheres"], stats["terrains"], args.math),
             "samples_per_step": scene.width * scene.height,
+            "schedule": "wavefront" if k_engine == dfa.PT_WAVEFRONT else "megakernel",
+            "autotune_passes": tune_passes,
             "rays_per_step": c["rays"] / steps,
             "samples_per_s": round(c["samples"] / wall, 1),
         },

@@ -201,11 +201,11 @@ struct DPaths {
   int32_t pad;
 };
 
-// Length of DPaths::ctl for a maximum depth: B + 1 live-path counts, B
-// shadow counts and 3B + 3 work heads, B = max(1, depth).
+// Length of DPaths::ctl for a maximum depth: B + 1 live-path counts, B + 1
+// shadow counts (the last stays zero) and 3B + 4 work heads, B = max(1, depth).
 __host__ __device__ inline int wavefront_ctl_len(int depth_max) {
   const int b = depth_max > 1 ? depth_max : 1;
-  return 5 * b + 4;
+  return 5 * b + 6;
 }
 
 }  // namespace dfk

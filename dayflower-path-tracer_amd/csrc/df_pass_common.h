@@ -70,7 +70,7 @@ __device__ void image_pass(const DParams& P, const float film[3], int prim_off, 
 
 constexpr int kCounterCount = 8;
 // DFrame::counters holds kCounterCount counters per stage (DF_STAGE_* order).
-enum Stage { kStageMega = 0, kStageRaygen, kStageClosest, kStageShade, kStageShadow, kStageEscape, kStageFilm,
+enum Stage { kStageMega = 0, kStageRaygen, kStageTrace, kStageShade, kStageEscape, kStageFilm,
              kStageCount };
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {

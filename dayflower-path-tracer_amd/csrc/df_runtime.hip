@@ -71,6 +71,12 @@ struct df_ctx {
   DevBuf path_f4[11], path_q[3], path_ctl;
   dfk::DPaths paths{};
 
+  // path-tracer schedule chosen by the autotuner for DF_PT_AUTO (0: not yet)
+  int tuned_engine = 0;
+  int tune_runs[2] = {0, 0};      // megakernel, wavefront passes timed
+  float tune_ms[2] = {0.0f, 0.0f};  // best time of one pass
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};
+
   // differentiable mode
   bool diff_on = false;
   int32_t diff_n_albedo = 0, diff_n_emission = 0;
@@ -649,6 +655,7 @@ int ensure_paths(df_ctx* c, int depth_max) {
   const size_t n = pixels(c);
   const int ctl_len = dfk::wavefront_ctl_len(depth_max);
   if (c->paths.n == static_cast<int64_t>(n) && c->paths.ctl_len >= ctl_len) return DF_OK;
+  DF_HIP(c, hipStreamSynchronize(c->stream));  // earlier launches may still read the old buffers
   free_paths(c);
   int rc;
   for (DevBuf& b : c->path_f4)
@@ -675,7 +682,25 @@ bool use_wavefront(const df_ctx* c) {
   if (c->params.renderer_type != DF_RENDERER_PATH_TRACER) return false;
   if (c->params.pt_engine == DF_PT_WAVEFRONT) return true;
   if (c->params.pt_engine == DF_PT_MEGAKERNEL) return false;
-  return pixels(c) >= (1u << 18);
+  return c->tuned_engine == DF_PT_WAVEFRONT;
+}
+
+// DF_PT_AUTO chooses by measurement: the first passes of a frame alternate
+// the two schedules, one pass each, until each has run kTunePasses times, and
+// the one with the shorter best pass is kept (both give the same bits, so
+// splitting passes between them changes nothing in the result).  Changing
+// the scene, the resolution or the parameters starts over.
+constexpr int kTunePasses = 2;
+
+bool tuning(const df_ctx* c) {
+  return c->params.renderer_type == DF_RENDERER_PATH_TRACER && c->params.pt_engine == DF_PT_AUTO &&
+         c->tuned_engine == 0;
+}
+
+void reset_tuning(df_ctx* c) {
+  c->tuned_engine = 0;
+  c->tune_runs[0] = c->tune_runs[1] = 0;
+  c->tune_ms[0] = c->tune_ms[1] = 0.0f;
 }
 
 hipEvent_t take_event(df_ctx* c) {
@@ -876,6 +901,8 @@ void df_destroy(df_ctx* c) {
                     &c->diff_weight, &c->diff_grad, &c->diff_loss, &c->diff_seeds})
     free_buf(*b);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : c->tune_ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -911,6 +938,7 @@ int df_load_scene(df_ctx* c, const df_compiled_scene* s) {
   }
   std::memcpy(c->camera, s->camera, sizeof c->camera);
   c->has_scene = true;
+  reset_tuning(c);
   return DF_OK;
 }
 
@@ -922,6 +950,7 @@ int df_update(df_ctx* c, int32_t w, int32_t h, int32_t local_size_hint) {
   DF_HIP(c, hipStreamSynchronize(c->stream));
   const size_t n = static_cast<size_t>(w) * h;
   free_paths(c);
+  reset_tuning(c);
   int rc;
   if ((rc = alloc_buf(c, c->film, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->film_samples, n * sizeof(int32_t))) ||
       (rc = alloc_buf(c, c->image_float, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->image_bgra, n * 4)) ||
@@ -1022,6 +1051,7 @@ int df_set_params(df_ctx* c, const df_params* p) {
   if (!c || !p) return DF_EINVAL;
   const int rc = validate_params(c, *p);
   if (rc) return rc;
+  if (std::memcmp(&c->params, p, sizeof *p) != 0) reset_tuning(c);
   c->params = *p;
   return DF_OK;
 }
@@ -1074,24 +1104,53 @@ int df_execute(df_ctx* c, int32_t passes) {
   DF_HIP(c, hipSetDevice(c->device));
   const DCamera cam = make_camera(c->camera);
   const DSky sky = make_sky(c->sky);
-  DParams par = make_params(c->params, c->width, c->height, passes, c->clear_pending);
-  apply_diff(c, par);
   const DFrame f = frame_of(c);
-  const bool wavefront = use_wavefront(c);
-  if (wavefront && (rc = ensure_paths(c, c->params.pt_depth_maximum))) return rc;
+  const bool may_wave = use_wavefront(c) || tuning(c);
+  if (may_wave && (rc = ensure_paths(c, c->params.pt_depth_maximum))) return rc;
   const dfk_marks marks{stage_mark, c};
   const dfk_marks* mk = c->stage_timing ? &marks : nullptr;
   c->mark_error = hipSuccess;
+  // `n` passes with one schedule; the film is cleared before the first if `clear`
+  auto run = [&](bool wave, int n, bool clear) -> int {
+    DParams par = make_params(c->params, c->width, c->height, n, clear);
+    apply_diff(c, par);
+    if (wave) {
+      DF_HIP(c, dfk_launch_pt_wavefront(c->ds, cam, sky, par, f, c->paths, c->params.math_mode, c->stream, mk));
+    } else {
+      DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, c->params.math_mode, c->stream, mk));
+    }
+    return DF_OK;
+  };
   DF_HIP(c, hipEventRecord(c->ev0, c->stream));
-  if (wavefront) {
-    DF_HIP(c, dfk_launch_pt_wavefront(c->ds, cam, sky, par, f, c->paths, c->params.math_mode, c->stream, mk));
-  } else {
-    DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, c->params.math_mode, c->stream, mk));
+  int done = 0;
+  while (done < passes && tuning(c)) {
+    for (hipEvent_t& e : c->tune_ev)
+      if (!e) DF_HIP(c, hipEventCreate(&e));
+    const int w = c->tune_runs[0] > c->tune_runs[1] ? 1 : 0;
+    DF_HIP(c, hipEventRecord(c->tune_ev[0], c->stream));
+    if ((rc = run(w == 1, 1, c->clear_pending && done == 0))) return rc;
+    DF_HIP(c, hipEventRecord(c->tune_ev[1], c->stream));
+    DF_HIP(c, hipEventSynchronize(c->tune_ev[1]));
+    float ms = 0.0f;
+    DF_HIP(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
+    c->tune_ms[w] = c->tune_runs[w] == 0 ? ms : std::min(c->tune_ms[w], ms);
+    ++c->tune_runs[w];
+    if (c->tune_runs[0] >= kTunePasses && c->tune_runs[1] >= kTunePasses)
+      c->tuned_engine = c->tune_ms[1] < c->tune_ms[0] ? DF_PT_WAVEFRONT : DF_PT_MEGAKERNEL;
+    ++done;
   }
+  if (done < passes && (rc = run(use_wavefront(c), passes - done, c->clear_pending && done == 0))) return rc;
   DF_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (c->mark_error != hipSuccess) return hip_fail(c, c->mark_error, "stage timing event");
   c->timed = true;
   c->clear_pending = false;  // clearFilmFlags() after the pass
+  return DF_OK;
+}
+
+int df_get_engine(df_ctx* c, int32_t* engine) {
+  if (!c || !engine) return DF_EINVAL;
+  if (c->params.pt_engine != DF_PT_AUTO) *engine = c->params.pt_engine;
+  else *engine = c->tuned_engine;
   return DF_OK;
 }
 
@@ -1272,9 +1331,13 @@ int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, d
       return fail(c, DF_EINVAL, "the adjoint needs Russian roulette off (pt_depth_russian_roulette > pt_depth_maximum)");
   }
   DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));  // nothing queued still uses the buffers below
   const size_t np = pixels(c);
-  if ((rc = alloc_buf(c, c->diff_weight, np * 3 * sizeof(float))) || (rc = alloc_buf(c, c->diff_loss, sizeof(double))) ||
-      (rc = alloc_buf(c, c->diff_seeds, np * sizeof(uint64_t))))
+  if (c->diff_weight.bytes != np * 3 * sizeof(float) &&
+      (rc = alloc_buf(c, c->diff_weight, np * 3 * sizeof(float))))
+    return rc;
+  if (!c->diff_loss.p && (rc = alloc_buf(c, c->diff_loss, sizeof(double)))) return rc;
+  if (c->diff_seeds.bytes != np * sizeof(uint64_t) && (rc = alloc_buf(c, c->diff_seeds, np * sizeof(uint64_t))))
     return rc;
   // the forward pass starts from these seeds; the replay starts from them again
   DF_HIP(c, hipMemcpyAsync(c->diff_seeds.p, c->seeds.p, np * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
@@ -1284,7 +1347,9 @@ int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, d
   DF_HIP(c, hipEventSynchronize(c->ev1));
   DF_HIP(c, hipEventElapsedTime(&fms, c->ev0, c->ev1));
   if (forward_ms) *forward_ms = fms;
-  DF_HIP(c, hipMemcpy(c->diff_weight.p, target, n * sizeof(float), hipMemcpyHostToDevice));
+  // every copy below is ordered on the renderer's stream (a plain hipMemcpy
+  // would not wait for it: the stream is non-blocking)
+  DF_HIP(c, hipMemcpyAsync(c->diff_weight.p, target, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
   DF_HIP(c, hipMemsetAsync(c->diff_loss.p, 0, sizeof(double), c->stream));
   // weights in place: dLoss/dfilm / passes
   DF_HIP(c, dfk_launch_loss(static_cast<const float*>(c->film.p), static_cast<const float*>(c->diff_weight.p),
@@ -1292,7 +1357,7 @@ int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, d
                             static_cast<double*>(c->diff_loss.p), c->stream));
   if (grad) {
     const size_t ng = n_grad;
-    if ((rc = alloc_buf(c, c->diff_grad, ng * sizeof(double)))) return rc;
+    if (c->diff_grad.bytes != ng * sizeof(double) && (rc = alloc_buf(c, c->diff_grad, ng * sizeof(double)))) return rc;
     DF_HIP(c, hipMemsetAsync(c->diff_grad.p, 0, ng * sizeof(double), c->stream));
     // replay from the forward's starting seeds; the replay leaves the seeds as the forward did
     DF_HIP(c, hipMemcpyAsync(c->seeds.p, c->diff_seeds.p, np * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
@@ -1315,10 +1380,11 @@ int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, d
     float ams = 0.0f;
     DF_HIP(c, hipEventElapsedTime(&ams, c->ev0, c->ev1));
     if (adjoint_ms) *adjoint_ms = ams;
-    DF_HIP(c, hipMemcpy(grad, c->diff_grad.p, ng * sizeof(double), hipMemcpyDeviceToHost));
+    DF_HIP(c, hipMemcpyAsync(grad, c->diff_grad.p, ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
   double l = 0.0;
-  DF_HIP(c, hipMemcpy(&l, c->diff_loss.p, sizeof(double), hipMemcpyDeviceToHost));
+  DF_HIP(c, hipMemcpyAsync(&l, c->diff_loss.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
   if (loss) *loss = l;
   return DF_OK;
 }

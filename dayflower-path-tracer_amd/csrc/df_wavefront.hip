@@ -7,15 +7,15 @@
 // kernels per stage, each over a queue of the paths still alive:
 //
 //   raygen           primary rays (camera, lens) for every pixel
-//   per bounce b:    shadow(b-1)  any-hit queries of the previous bounce's
-//                                 sun samples, resolving its radiance
-//                    closest(b)   closest-hit queries of the live paths
+//   per bounce b:    trace(b)     any-hit queries of the previous bounce's
+//                                 sun samples (resolving its radiance), then
+//                                 closest-hit queries of the live paths
 //                    shade(b)     hit record, emission, sun sample, Russian
 //                                 roulette, scattered ray; appends the path
 //                                 to the next bounce's queue and its sun
 //                                 sample to the shadow queue
-//   shadow(last), escape          the sky test of paths that reached the
-//                                 maximum depth
+//   trace            the last bounce's sun samples
+//   escape           the sky test of the paths that reached the maximum depth
 //   film             film accumulation, wireframes and the image pass
 //
 // Every path draws from its pixel's LCG stream in the reference's order
@@ -106,24 +106,6 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
 }
 
 template <int M>
-__global__ __launch_bounds__(kThreads) void wf_closest(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
-                                                       const int32_t* q, const int32_t* count, int32_t* head,
-                                                       int first_bounce) {
-  Tracer<M> tr(S, C, K, P, kPerm);
-  drain(q, count, head, [&](bool valid, int32_t i) {
-    if (!valid) return;
-    const float4 o = W.ray_o[i], d = W.ray_d[i];
-    int ci, cst, cso;
-    const float t = tr.template query<false>(o.x, o.y, o.z, d.x, d.y, d.z, ci, cst, cso);
-    W.hit[i] = make_float4(t, __int_as_float(ci), __int_as_float(cst), __int_as_float(cso));
-    if (first_bounce) F.primitive_offsets[i] = ci == -1 ? -1 : ci * 3;
-  });
-  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
-  take_counters(tr, cnt);
-  flush_counters(cnt, F.counters + kStageClosest * kCounterCount);
-}
-
-template <int M>
 __global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
@@ -168,11 +150,23 @@ __global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K
   flush_counters(cnt, F.counters + kStageShade * kCounterCount);
 }
 
-template <int M>
-__global__ __launch_bounds__(kThreads) void wf_shadow(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
-                                                      const int32_t* count, int32_t* head) {
+// The ray queries of one bounce in one launch: first the sun's shadow
+// queries of the previous bounce (any-hit; each resolves that bounce's
+// radiance, and finishes the path if it ended there), then the closest-hit
+// queries of the live paths - or, after the last bounce, their escape tests
+// (any-hit + sky, :3689-3700).  Both queues drain through the same
+// persistent waves, so a short queue does not leave the GPU idle and a bounce
+// pays one launch tail.  The two queues of one launch never touch the same
+// path data: a shadow query writes the radiance and the sample, a
+// closest-hit query the hit record; an escape test reads the radiance, so
+// the last sun queries run in a launch before it.
+template <int M, bool kEscape>
+__global__ __launch_bounds__(kThreads) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+                                                     const int32_t* shadow_count, int32_t* shadow_head,
+                                                     const int32_t* q, const int32_t* count, int32_t* head,
+                                                     int first_bounce) {
   Tracer<M> tr(S, C, K, P, kPerm);
-  drain(W.shadow_queue, count, head, [&](bool valid, int32_t e) {
+  drain(W.shadow_queue, shadow_count, shadow_head, [&](bool valid, int32_t e) {
     if (!valid) return;
     const int32_t i = e & (kFinish - 1);
     const float4 o = W.sh_o[i], d = W.sh_d[i];
@@ -188,27 +182,26 @@ __global__ __launch_bounds__(kThreads) void wf_shadow(DScene S, DCamera C, DSky 
       W.sample[i] = make_float4(p.x, p.y, p.z, 0.0f);
     }
   });
-  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
-  take_counters(tr, cnt);
-  flush_counters(cnt, F.counters + kStageShadow * kCounterCount);
-}
-
-template <int M>
-__global__ __launch_bounds__(kThreads) void wf_escape(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
-                                                      const int32_t* q, const int32_t* count, int32_t* head) {
-  Tracer<M> tr(S, C, K, P, kPerm);
   const bool sky_active = K.sky_active == 1;
   drain(q, count, head, [&](bool valid, int32_t i) {
     if (!valid) return;
-    const float4 o4 = W.ray_o[i], d4 = W.ray_d[i], m4 = W.thru[i], p4 = W.rad[i];
-    const float o[3] = {o4.x, o4.y, o4.z}, d[3] = {d4.x, d4.y, d4.z}, m[3] = {m4.x, m4.y, m4.z};
-    float p[3] = {p4.x, p4.y, p4.z};
-    tr.pt_escape(o, d, m, p, sky_active);
-    W.sample[i] = f4(p, 0.0f);
+    const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
+    if constexpr (kEscape) {
+      const float4 m4 = W.thru[i], p4 = W.rad[i];
+      const float o[3] = {o4.x, o4.y, o4.z}, d[3] = {d4.x, d4.y, d4.z}, m[3] = {m4.x, m4.y, m4.z};
+      float p[3] = {p4.x, p4.y, p4.z};
+      tr.pt_escape(o, d, m, p, sky_active);
+      W.sample[i] = f4(p, 0.0f);
+    } else {
+      int ci, cst, cso;
+      const float t = tr.template query<false>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z, ci, cst, cso);
+      W.hit[i] = make_float4(t, __int_as_float(ci), __int_as_float(cst), __int_as_float(cso));
+      if (first_bounce) F.primitive_offsets[i] = ci == -1 ? -1 : ci * 3;
+    }
   });
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   take_counters(tr, cnt);
-  flush_counters(cnt, F.counters + kStageEscape * kCounterCount);
+  flush_counters(cnt, F.counters + (kEscape ? kStageEscape : kStageTrace) * kCounterCount);
 }
 
 // filmAddColor / filmSetColor, the wireframe overlay and, after the last pass,
@@ -273,18 +266,17 @@ int resident_blocks(Kernel k) {
 template <int M>
 hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
                             const DPaths& w, hipStream_t stream, const dfk_marks* marks) {
-  static const int g_closest = resident_blocks(wf_closest<M>);
+  static const int g_trace = resident_blocks(wf_trace<M, false>);
   static const int g_shade = resident_blocks(wf_shade<M>);
-  static const int g_shadow = resident_blocks(wf_shadow<M>);
-  static const int g_escape = resident_blocks(wf_escape<M>);
+  static const int g_escape = resident_blocks(wf_trace<M, true>);
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
-  // ctl: [0, B] live-path counts per bounce (B = escape queue), [B+1, 2B] shadow
-  // counts, then one work head per trace launch.
+  // ctl: [0, B] live-path counts per bounce (B = escape queue), [B+1, 2B]
+  // shadow counts (plus one that stays zero), then the work heads.
   int32_t* ctl = w.ctl;
   int32_t* shadow_count = ctl + bounces + 1;
-  int32_t* heads = ctl + 2 * bounces + 1;
+  int32_t* heads = ctl + 2 * bounces + 2;
   const dim3 pix_block(16, 16);
   const dim3 pix_grid((p.width + 15) / 16, (p.height + 15) / 16);
   auto mark = [&](int stage, int begin) {
@@ -296,28 +288,29 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
     hipLaunchKernelGGL(wf_raygen<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w);
     mark(kStageRaygen, 0);
     for (int b = 0; b < bounces; ++b) {
-      if (b > 0) {
-        mark(kStageShadow, 1);
-        hipLaunchKernelGGL(wf_shadow<M>, dim3(g_shadow), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                           shadow_count + (b - 1), heads + 3 * b + 2);
-        mark(kStageShadow, 0);
-      }
-      mark(kStageClosest, 1);
-      hipLaunchKernelGGL(wf_closest<M>, dim3(g_closest), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
-                         ctl + b, heads + 3 * b, b == 0 ? 1 : 0);
-      mark(kStageClosest, 0);
+      // shadow queries of bounce b - 1 (none before the first bounce: that count stays 0)
+      int32_t* sc = b > 0 ? shadow_count + (b - 1) : shadow_count + bounces;
+      mark(kStageTrace, 1);
+      hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
+                         heads + 3 * b, w.queue[b & 1], ctl + b, heads + 3 * b + 1, b == 0 ? 1 : 0);
+      mark(kStageTrace, 0);
       mark(kStageShade, 1);
       hipLaunchKernelGGL(wf_shade<M>, dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
-                         ctl + b, heads + 3 * b + 1, w.queue[(b + 1) & 1], ctl + b + 1, shadow_count + b, b);
+                         ctl + b, heads + 3 * b + 2, w.queue[(b + 1) & 1], ctl + b + 1, shadow_count + b, b);
       mark(kStageShade, 0);
     }
-    mark(kStageShadow, 1);
-    hipLaunchKernelGGL(wf_shadow<M>, dim3(g_shadow), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                       shadow_count + (bounces - 1), heads + 3 * bounces + 2);
-    mark(kStageShadow, 0);
+    // the last bounce's sun queries resolve radiance that the escape tests
+    // then add to, so they finish first (a launch of their own: its
+    // closest-hit queue is the zero count)
+    mark(kStageTrace, 1);
+    hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                       shadow_count + (bounces - 1), heads + 3 * bounces, w.queue[bounces & 1],
+                       shadow_count + bounces, heads + 3 * bounces + 1, 0);
+    mark(kStageTrace, 0);
     mark(kStageEscape, 1);
-    hipLaunchKernelGGL(wf_escape<M>, dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                       w.queue[bounces & 1], ctl + bounces, heads + 3 * bounces);
+    hipLaunchKernelGGL((wf_trace<M, true>), dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                       shadow_count + bounces, heads + 3 * bounces + 2, w.queue[bounces & 1], ctl + bounces,
+                       heads + 3 * bounces + 3, 0);
     mark(kStageEscape, 0);
     mark(kStageFilm, 1);
     hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w,

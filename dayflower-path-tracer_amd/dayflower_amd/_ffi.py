@@ -110,7 +110,7 @@ class Params(ctypes.Structure):
     ]
 
 
-STAGE_NAMES = ("megakernel", "raygen", "closest", "shade", "shadow", "escape", "film")
+STAGE_NAMES = ("megakernel", "raygen", "trace", "shade", "escape", "film")
 
 COUNTER_NAMES = ("rays", "samples", "box_tests", "tri_tests", "prim_tests", "sphere_tests", "terrain_steps",
                  "sky_evals")
@@ -150,6 +150,7 @@ HIP_PROTOTYPES = {
     "df_clear": (ctypes.c_int, [_P]),
     "df_execute": (ctypes.c_int, [_P, ctypes.c_int32]),
     "df_synchronize": (ctypes.c_int, [_P]),
+    "df_get_engine": (ctypes.c_int, [_P, c_int32_p]),
     "df_last_execute_ms": (ctypes.c_int, [_P, c_float_p]),
     "df_get_image_bgra": (ctypes.c_int, [_P, c_uint8_p, ctypes.c_size_t]),
     "df_get_image_float": (ctypes.c_int, [_P, c_float_p, ctypes.c_size_t]),

@@ -401,6 +401,12 @@ class HIPRendererKernel:
         self._params.pt_engine = int(engine)
         self._push_params()
 
+    def engine(self):
+        """PT_MEGAKERNEL or PT_WAVEFRONT, or PT_AUTO while the autotuner is still timing both."""
+        e = ctypes.c_int32()
+        self._check(self._lib.df_get_engine(self._ctx, ctypes.byref(e)), "df_get_engine")
+        return e.value
+
     def setParams(self, params):
         ctypes.memmove(ctypes.byref(self._params), ctypes.byref(params), ctypes.sizeof(_ffi.Params))
         self._push_params()

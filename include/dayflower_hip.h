@@ -104,6 +104,11 @@ int df_clear(df_ctx* ctx);
  * pixel per pass.  Asynchronous on the handle's stream. */
 int df_execute(df_ctx* ctx, int32_t passes);
 
+/* The path tracer's schedule for the next df_execute: DF_PT_MEGAKERNEL or
+ * DF_PT_WAVEFRONT, or DF_PT_AUTO while the autotuner is still timing the
+ * two on the current frame (no counterpart in the reference). */
+int df_get_engine(df_ctx* ctx, int32_t* engine);
+
 /* Waits for the handle's stream. */
 int df_synchronize(df_ctx* ctx);
 

@@ -59,8 +59,9 @@ enum { DF_SHADER_FLAT = 1, DF_SHADER_GOURAUD = 2 };
 enum { DF_MATH_FAST = 0, DF_MATH_EXACT = 1 };
 
 /* How the path tracer is scheduled on the GPU; every choice gives the same
- * bits.  AUTO picks the wavefront kernels for frames of at least 2^18 pixels
- * and the megakernel below that.
+ * bits.  AUTO times both schedules on the first passes of a frame (one pass
+ * at a time, alternating, two passes each) and keeps the faster one until
+ * the scene, resolution or parameters change.
  * DF_PT_MEGAKERNEL : one work-item runs a pixel's whole path.
  * DF_PT_WAVEFRONT  : a kernel per stage (closest-hit, shade, shadow) over
  *                    queues of live paths. */
@@ -72,12 +73,11 @@ enum { DF_PT_AUTO = 0, DF_PT_MEGAKERNEL = 1, DF_PT_WAVEFRONT = 2 };
 enum {
   DF_STAGE_MEGAKERNEL = 0,
   DF_STAGE_RAYGEN = 1,
-  DF_STAGE_CLOSEST = 2,   /* closest-hit queries of live paths          */
-  DF_STAGE_SHADE = 3,     /* hit record, textures, sun sample, BSDF      */
-  DF_STAGE_SHADOW = 4,    /* any-hit queries towards the sun            */
-  DF_STAGE_ESCAPE = 5,    /* any-hit + sky after the maximum depth      */
-  DF_STAGE_FILM = 6,      /* film update, wireframes, image pass        */
-  DF_STAGE_COUNT = 7
+  DF_STAGE_TRACE = 2,     /* sun any-hit queries of a bounce, then the next closest-hit queries */
+  DF_STAGE_SHADE = 3,     /* hit record, textures, sun sample, BSDF        */
+  DF_STAGE_ESCAPE = 4,    /* any-hit + sky after the maximum depth         */
+  DF_STAGE_FILM = 5,      /* film update, wireframes, image pass           */
+  DF_STAGE_COUNT = 6
 };
 
 /* The 14 flat arrays of a compiled scene plus the camera, as (pointer,

@@ -354,3 +354,30 @@ def test_error_paths(scenes, gpu):
         assert k.getRendererType() == 2
     finally:
         k.dispose()
+
+
+@pytest.mark.parametrize("cid,w,h", [(2, 128, 96), (3, 160, 90)])
+def test_autotuned_schedule_gives_the_same_bits(scenes, gpu, cid, w, h):
+    """PT_AUTO alternates the megakernel and the wavefront schedule one pass at a time while it times them, then
+    keeps one: the film equals a megakernel-only render of the same passes."""
+    sc = _small(scenes(cid), w, h)
+    out = []
+    for engine in (dfa.PT_AUTO, dfa.PT_MEGAKERNEL):
+        p = sc.copy_params()
+        p.math_mode = dfa.MATH_FAST
+        p.pt_engine = engine
+        k = dfa.HIPRendererKernel(sc)
+        try:
+            k.setParams(p)
+            k.update(w, h)
+            k.seed(11)
+            k.execute(3)  # tuning: one pass at a time
+            k.execute(4)
+            out.append((k.getFilm(), k.getSeeds(), k.engine()))
+        finally:
+            k.dispose()
+    (fa, sa, ea), (fb, sb, eb) = out
+    np.testing.assert_array_equal(fa[0], fb[0])
+    np.testing.assert_array_equal(fa[1], fb[1])
+    np.testing.assert_array_equal(sa, sb)
+    assert ea in (dfa.PT_MEGAKERNEL, dfa.PT_WAVEFRONT) and eb == dfa.PT_MEGAKERNEL
