@@ -17,6 +17,9 @@ Besides the contract fields the JSON line carries:
   batch        - the batch mix (256 scenes of 64^2..512^2 pixels and 8..128 samples per
                  pixel): Mrays/s rendering the scenes one after another (plain)
                  and in one batched launch, and their ratio;
+  adjoint      - the differentiation workload (512^2, 32 spp, 347,680 triangles):
+                 device time of the forward render and of the path-replay
+                 adjoint, and their ratio;
   cpu_baseline - the CPU oracle (the reference's per-pixel kernel restated in
                  C++ with the reference's thread-pool semantics) on a band of
                  rows of the same frame, on this host's cores.
@@ -140,6 +143,49 @@ def traffic_from_profiles(workload, kernel_symbol):
     return k["hbm_bytes_per_dispatch"] if k else None
 
 
+def adjoint_measure():
+    """The differentiation workload (512x512, 32 samples per pixel, depth 6, 347,680 triangles): the forward
+    render and the path-replay adjoint of L = sum (film - target)^2 with respect to 64 RGB albedo groups, 4
+    emissions, turbidity and sun elevation (target: a render at the hidden parameters of seed 500; current
+    parameters from seed 501), device time of each, after the schedule autotuner has settled."""
+    import numpy as np
+
+    import dayflower_amd as dfa
+
+    sc = dfa.build_config(5)
+    sa, se, na, ne = dfa.diff_maps(sc)
+
+    def draw(seed):
+        r = np.random.default_rng(seed)
+        return (r.uniform(0.1, 0.9, (na, 3)).astype(np.float32), r.uniform(1.0, 20.0, ne).astype(np.float32),
+                float(r.uniform(2.0, 6.0)), float(r.uniform(10.0, 70.0)))
+
+    k = dfa.HIPRendererKernel(sc)
+    try:
+        k.setParams(sc.copy_params())  # depth 6, Russian roulette off
+        k.update(sc.width, sc.height)
+        a, e, t, el = draw(500)
+        k.setDifferentiable(sa, se, a, e, t, el, 30.0)
+        k.seed(5)
+        k.lossGradient(np.zeros(sc.width * sc.height * 3, np.float32), sc.spp, gradient=False)
+        target = k.getFilm()[0].copy()
+        a, e, t, el = draw(501)
+        k.setDifferentiable(sa, se, a, e, t, el, 30.0)
+        runs = []
+        for _ in range(3):
+            k.seed(7)
+            loss, g, fms, ams = k.lossGradient(target, sc.spp, gradient=True)
+            runs.append((fms, ams))
+        engine = k.engine()
+    finally:
+        k.dispose()
+    fms, ams = runs[-1]
+    return {"workload": "512x512, %d spp, depth 6, %d triangles, %d parameters" % (sc.spp, dfa.scene.stats(sc)["triangles"],
+                                                                                   g.size),
+            "forward_ms": round(fms, 3), "adjoint_ms": round(ams, 3), "adjoint_over_forward": round(ams / fms, 3),
+            "schedule": "wavefront" if engine == dfa.PT_WAVEFRONT else "megakernel", "loss": loss}
+
+
 def batch_measure(n_scenes):
     """The uneven batch mix (scene seeds 100.., each at its own resolution and samples per pixel): every scene
     rendered alone one after another (plain, the best single-scene schedule), then all in one batched launch."""
@@ -184,6 +230,7 @@ def main():
     ap.add_argument("--math", choices=["fast", "exact"], default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint / forward time measurement")
     ap.add_argument("--batch-scenes", type=int, default=256,
                     help="scenes of the batch mix (seeds 100, 101, ...) rendered plain and batched; 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -287,6 +334,7 @@ def main():
                      for n in stages if stages[n]["launches"] > 0}
 
     batch = batch_measure(args.batch_scenes) if args.batch_scenes > 0 else None
+    adjoint = None if args.no_adjoint else adjoint_measure()
 
     cpu = None
     if not args.no_cpu:
@@ -321,6 +369,7 @@ def main():
         "roofline": roofline,
         "stages": stage_summary,
         "batch": batch,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
