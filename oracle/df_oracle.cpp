@@ -152,6 +152,266 @@ void daylight_xyy_to_srgb(float fx, float fy, double lum, float* r_out, float* g
   *b_out = b + w;
 }
 
+// ------------------------------------------------------------ FP64 adjoint
+//
+// Forward-mode differentiation of the path tracer in double precision, for
+// the differentiable mode (continuous albedo per surface group, emission per
+// light, the sky's turbidity and the sun's elevation).  Every path is traced
+// by render_pt exactly as in the forward render (same floats, same random
+// numbers, Russian roulette off); beside it a tangent carries the derivative
+// of the path throughput and the sample with respect to every parameter,
+// in double.  Geometry and visibility do not move with the parameters, so
+// this is the interior term of the gradient.  The sky's derivatives come
+// from dual numbers through Sky.set and the Perez evaluation, the sun's
+// from dual numbers through the sun cone sample at the forward's random
+// numbers: no finite differences anywhere.
+
+// A value with its derivatives with respect to turbidity (t) and sun
+// elevation in degrees (e).
+struct D2 {
+  double v, t, e;
+  D2(double value = 0.0, double dt = 0.0, double de = 0.0) : v(value), t(dt), e(de) {}
+};
+inline D2 operator+(D2 a, D2 b) { return D2(a.v + b.v, a.t + b.t, a.e + b.e); }
+inline D2 operator-(D2 a, D2 b) { return D2(a.v - b.v, a.t - b.t, a.e - b.e); }
+inline D2 operator-(D2 a) { return D2(-a.v, -a.t, -a.e); }
+inline D2 operator*(D2 a, D2 b) { return D2(a.v * b.v, a.t * b.v + a.v * b.t, a.e * b.v + a.v * b.e); }
+inline D2 operator/(D2 a, D2 b) {
+  const double q = a.v / b.v;
+  return D2(q, (a.t - q * b.t) / b.v, (a.e - q * b.e) / b.v);
+}
+inline D2 chain(D2 a, double f, double df) { return D2(f, df * a.t, df * a.e); }
+inline D2 dexp(D2 a) {
+  const double f = std::exp(a.v);
+  return chain(a, f, f);
+}
+inline D2 dsin(D2 a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
+inline D2 dcos(D2 a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
+inline D2 dtan(D2 a) {
+  const double c = std::cos(a.v);
+  return chain(a, std::tan(a.v), 1.0 / (c * c));
+}
+inline D2 dsqrt(D2 a) {  // at zero: derivative zero (the cone's centre, where 1 - cos^2 stays zero)
+  if (!(a.v > 0.0)) return D2(0.0);
+  const double f = std::sqrt(a.v);
+  return chain(a, f, 0.5 / f);
+}
+inline D2 dacos_clamped(D2 a) {  // acos(max(min(a, 1), -1)): constant outside [-1, 1]
+  if (a.v >= 1.0) return D2(0.0);
+  if (a.v <= -1.0) return D2(M_PI);
+  return chain(a, std::acos(a.v), -1.0 / std::sqrt(1.0 - a.v * a.v));
+}
+inline D2 dmax0(D2 a) { return a.v > 0.0 ? a : D2(0.0); }
+
+// Sky.set (Sky.java:295-333) as a function of turbidity T and sun elevation
+// e (degrees) at a fixed azimuth a, the sun towards (cos e sin a, sin e,
+// cos e cos a) in the world (y up, azimuth from +z towards +x).  The sky
+// frame is the one the forward's sky holds (u = x, v = z, w = y).
+struct DualSky {
+  D2 world[3];  // unit vector towards the sun, world frame
+  D2 dir[3];    // the same in the sky frame
+  D2 theta;     // the sun's zenith angle
+  D2 zl, zx, zy;
+  D2 L[5], X[5], Y[5];
+  float onb_u[3], onb_v[3], onb_w[3];
+};
+
+DualSky dual_sky(const df_sky* frame, double turbidity, double elevation_deg, double azimuth_deg) {
+  DualSky K;
+  std::memcpy(K.onb_u, frame->onb_u, sizeof K.onb_u);
+  std::memcpy(K.onb_v, frame->onb_v, sizeof K.onb_v);
+  std::memcpy(K.onb_w, frame->onb_w, sizeof K.onb_w);
+  const double rad = M_PI / 180.0;
+  const D2 el(elevation_deg * rad, 0.0, rad);
+  const D2 T(turbidity, 1.0, 0.0);
+  const double sa = std::sin(azimuth_deg * rad), ca = std::cos(azimuth_deg * rad);
+  K.world[0] = dcos(el) * sa;
+  K.world[1] = dsin(el);
+  K.world[2] = dcos(el) * ca;
+  const float* onb[3] = {K.onb_u, K.onb_v, K.onb_w};
+  for (int i = 0; i < 3; ++i) K.dir[i] = K.world[0] * onb[i][0] + K.world[1] * onb[i][1] + K.world[2] * onb[i][2];
+  const D2 len = dsqrt(K.dir[0] * K.dir[0] + K.dir[1] * K.dir[1] + K.dir[2] * K.dir[2]);
+  for (D2& c : K.dir) c = c / len;
+  K.theta = dacos_clamped(K.dir[2]);
+  const D2 th = K.theta, th2 = th * th, th3 = th2 * th, T2 = T * T;
+  const double pi_f = static_cast<double>(static_cast<float>(M_PI));  // MathF.PI
+  const D2 chi = (D2(4.0 / 9.0) - T / D2(120.0)) * (D2(pi_f) - D2(2.0) * th);
+  K.zl = ((D2(4.0453) * T - D2(4.9710)) * dtan(chi) - D2(0.2155) * T + D2(2.4192)) * D2(1000.0);
+  auto poly = [&](const double c[12]) {  // (c0 th3 + c1 th2 + c2 th + c3) T2 + (...) T + (...)
+    D2 r(0.0);
+    const D2 coef[3] = {T2, T, D2(1.0)};
+    for (int k = 0; k < 3; ++k)
+      r = r + (D2(c[4 * k]) * th3 + D2(c[4 * k + 1]) * th2 + D2(c[4 * k + 2]) * th + D2(c[4 * k + 3])) * coef[k];
+    return r;
+  };
+  const double cx[12] = {0.00165, -0.00374, 0.00208, 0.0, -0.02902, 0.06377, -0.03202, 0.00394,
+                         0.11693, -0.21196, 0.06052, 0.25885};
+  const double cy[12] = {0.00275, -0.00610, 0.00316, 0.0, -0.04212, 0.08970, -0.04153, 0.00515,
+                         0.15346, -0.26756, 0.06669, 0.26688};
+  K.zx = poly(cx);
+  K.zy = poly(cy);
+  // Perez coefficients, linear in T (Preetham et al. 1999 fits, Sky.java:320-334)
+  const double pl[5][2] = {{0.17872, -1.46303}, {-0.35540, 0.42749}, {-0.02266, 5.32505}, {0.12064, -2.57705},
+                           {-0.06696, 0.37027}};
+  const double px[5][2] = {{-0.01925, -0.25922}, {-0.06651, 0.00081}, {-0.00041, 0.21247}, {-0.06409, -0.89887},
+                           {-0.00325, 0.04517}};
+  const double py[5][2] = {{-0.01669, -0.26078}, {-0.09495, 0.00921}, {-0.00792, 0.21023}, {-0.04405, -1.65369},
+                           {-0.01092, 0.05291}};
+  for (int i = 0; i < 5; ++i) {
+    K.L[i] = D2(pl[i][0]) * T + D2(pl[i][1]);
+    K.X[i] = D2(px[i][0]) * T + D2(px[i][1]);
+    K.Y[i] = D2(py[i][0]) * T + D2(py[i][1]);
+  }
+  return K;
+}
+
+// doCalculateColorForSky (GPURendererKernel.java:2127-2236) in dual numbers
+// for the direction (dx, dy, dz), which does not depend on the parameters.
+// Returns false below the horizon, where the sky is a constant.
+bool dual_sky_rgb(const DualSky& K, float dx, float dy, float dz, D2 out[3]) {
+  const float d0X = dx * K.onb_u[0] + dy * K.onb_u[1] + dz * K.onb_u[2];
+  const float d0Y = dx * K.onb_v[0] + dy * K.onb_v[1] + dz * K.onb_v[2];
+  float d0Z = dx * K.onb_w[0] + dy * K.onb_w[1] + dz * K.onb_w[2];
+  if (!(d0Z >= 0.0f)) return false;
+  d0Z = jmaxf(d0Z, 0.001f);
+  const double l = 1.0 / std::sqrt(static_cast<double>(d0X) * d0X + static_cast<double>(d0Y) * d0Y +
+                                   static_cast<double>(d0Z) * d0Z);
+  const double d1X = d0X * l, d1Y = d0Y * l, d1Z = d0Z * l;
+  const D2 dp = K.dir[0] * D2(d1X) + K.dir[1] * D2(d1Y) + K.dir[2] * D2(d1Z);
+  const double theta1 = std::acos(jmaxd(jmind(d1Z, 1.0), -1.0));
+  const double cosTheta1R = 1.0 / (std::cos(theta1) + 0.01);
+  const D2 gamma = dacos_clamped(dp);
+  const D2 cosGamma = dcos(gamma);
+  const D2 cosTheta0 = dcos(K.theta);
+  const D2 one(1.0);
+  auto f_sun = [&](const D2* C, D2 angle) {  // 1 + C2 exp(C3 angle) + C4 cos^2
+    return one + C[2] * dexp(C[3] * angle) + C[4] * cosGamma * cosGamma;
+  };
+  const D2* L = K.L;
+  const D2* X = K.X;
+  const D2* Y = K.Y;
+  const D2 lden = (one + L[0] * dexp(L[1])) * (one + L[2] * dexp(L[3] * K.theta) + L[4] * cosTheta0 * cosTheta0);
+  const D2 lnum = (one + L[0] * dexp(L[1] * D2(cosTheta1R))) * f_sun(L, gamma);
+  const D2 lum = K.zl * lnum / lden * D2(1.0e-4);
+  const D2 xden = (one + X[0] * dexp(X[1])) * (one + X[2] * dexp(X[3] * D2(theta1)) + X[4] * cosTheta0 * cosTheta0);
+  const D2 xnum = (one + X[0] * dexp(X[1] * D2(cosTheta1R))) * f_sun(X, gamma);
+  const D2 x = K.zx * xnum / xden;
+  const D2 yden = (one + Y[0] * dexp(Y[1])) * (one + Y[2] * dexp(Y[3] * D2(theta1)) + Y[4] * cosTheta0 * cosTheta0);
+  const D2 ynum = (one + Y[0] * dexp(Y[1] * D2(cosTheta1R))) * f_sun(Y, gamma);
+  const D2 y = K.zy * ynum / yden;
+  // daylight chromaticity -> linear sRGB (GPURendererKernel.java:2207-2230)
+  const D2 cie0 = one / (D2(0.0241) + D2(0.2562) * x - D2(0.7341) * y);
+  const D2 cie1 = (D2(-1.3515) - D2(1.7703) * x + D2(5.9114) * y) * cie0;
+  const D2 cie2 = (D2(0.03) - D2(31.4424) * x + D2(30.0717) * y) * cie0;
+  const D2 cX = D2(10246.121) + cie1 * D2(187.75537) + cie2 * D2(213.14803);
+  const D2 cY = D2(10676.695) + cie1 * D2(192.59653) + cie2 * D2(76.29494);
+  const D2 cZ = D2(12372.504) + cie1 * D2(3482.8765) + cie2 * D2(-235.71611);
+  const D2 cR = cX * lum / cY, cG = lum, cB = cZ * lum / cY;
+  const D2 r = D2(3.2410042) * cR + D2(-1.5373994) * cG + D2(-0.49861607) * cB;
+  const D2 g = D2(-0.9692241) * cR + D2(1.8759298) * cG + D2(0.041554242) * cB;
+  const D2 b = D2(0.05563942) * cR + D2(-0.20401107) * cG + D2(1.0571486) * cB;
+  // w = max(0, -min(0, min(r, g, b))): the smallest channel lifted to zero
+  const D2* lo = &r;
+  if (g.v < lo->v) lo = &g;
+  if (b.v < lo->v) lo = &b;
+  const D2 w = lo->v < 0.0 ? -*lo : D2(0.0);
+  out[0] = r + w;
+  out[1] = g + w;
+  out[2] = b + w;
+  return true;
+}
+
+// The cosine dp1 between the normal (nX, nY, nZ) and the sun direction that
+// doCalculateColorForSun (GPURendererKernel.java:2238-2347) samples with the
+// random numbers (u, v) at (pX, pY, pZ), as a function of the sun's
+// elevation (the sun's origin sits 10000 units along its direction).
+D2 dual_sun_cosine(const DualSky& K, float pX, float pY, float pZ, float nX, float nY, float nZ, float u, float v) {
+  const D2* w = K.world;
+  const D2 toX = w[0] * D2(10000.0) - D2(pX), toY = w[1] * D2(10000.0) - D2(pY), toZ = w[2] * D2(10000.0) - D2(pZ);
+  const D2 to2 = toX * toX + toY * toY + toZ * toZ;
+  const D2 sinMax2 = D2(500.0 * 500.0) / to2;
+  const D2 cosMax = dsqrt(dmax0(D2(1.0) - sinMax2));
+  const D2 cosT = D2(u) * (cosMax - D2(1.0)) + D2(1.0);
+  const D2 sinT = dsqrt(dmax0(D2(1.0) - cosT * cosT));
+  const float phi = PI_MULTIPLIED_BY_TWO * v;
+  const D2 x = D2(std::cos(static_cast<double>(phi))) * sinT;
+  const D2 y = D2(std::sin(static_cast<double>(phi))) * sinT;
+  const D2 z = cosT;
+  const double aX = std::fabs(w[0].v), aY = std::fabs(w[1].v), aZ = std::fabs(w[2].v);
+  const bool c0 = aX < aY && aX < aZ;
+  const bool c1 = aY < aZ;
+  const D2 vX = c0 ? D2(0.0) : c1 ? w[2] : w[1];
+  const D2 vY = c0 ? w[2] : c1 ? D2(0.0) : -w[0];
+  const D2 vZ = c0 ? -w[1] : c1 ? -w[0] : D2(0.0);
+  const D2 vl = dsqrt(vX * vX + vY * vY + vZ * vZ);
+  const D2 vnX = vX / vl, vnY = vY / vl, vnZ = vZ / vl;
+  const D2 unX = vnY * w[2] - vnZ * w[1], unY = vnZ * w[0] - vnX * w[2], unZ = vnX * w[1] - vnY * w[0];
+  const D2 rX = unX * x + vnX * y + w[0] * z;
+  const D2 rY = unY * x + vnY * y + w[1] * z;
+  const D2 rZ = unZ * x + vnZ * y + w[2] * z;
+  const D2 rl = dsqrt(rX * rX + rY * rY + rZ * rZ);
+  return (rX * D2(nX) + rY * D2(nY) + rZ * D2(nZ)) / rl;
+}
+
+// Differentiable mode: the surface records mapped to a parameter take it in
+// place of their constant texture.
+struct DiffMaps {
+  int n_albedo, n_emission;
+  const int32_t* surface_albedo;
+  const int32_t* surface_emission;
+  int n_surfaces;
+  const float* albedo;
+  const float* emission;
+};
+
+// The tangent of one path: its throughput in double, the derivative of the
+// throughput with respect to the albedo groups the path has met so far
+// (the only parameters it depends on), and the pixel's running sum of the
+// derivatives of its samples, over all parameters.
+struct PathTangent {
+  static constexpr int kMaxSlots = 64;
+  const DualSky* sky;
+  int n_albedo, i_emission, i_turbidity, i_elevation;
+  double m[3];
+  int slots[kMaxSlots];
+  double dm[kMaxSlots][3];
+  int n_slots;
+  double* dpix;  // (3 n_albedo + n_emission + 2) x 3
+
+  void begin() {
+    m[0] = m[1] = m[2] = 1.0;
+    n_slots = 0;
+  }
+  int slot_row(int slot) {
+    for (int k = 0; k < n_slots; ++k)
+      if (slots[k] == slot) return k;
+    slots[n_slots] = slot;
+    dm[n_slots][0] = dm[n_slots][1] = dm[n_slots][2] = 0.0;
+    return n_slots++;
+  }
+  // d(sample) += d(m) * g: the term m * g with g not depending on the albedos
+  void through_throughput(const double g[3]) {
+    for (int k = 0; k < n_slots; ++k)
+      for (int c = 0; c < 3; ++c) dpix[(slots[k] * 3 + c) * 3 + c] += dm[k][c] * g[c];
+  }
+  // m *= s, s free of parameters
+  void scale(float s) {
+    for (int c = 0; c < 3; ++c) m[c] *= s;
+    for (int k = 0; k < n_slots; ++k)
+      for (int c = 0; c < 3; ++c) dm[k][c] *= s;
+  }
+  // m *= a, a the albedo of group `slot` (-1: a texture, no parameter)
+  void albedo(int slot, const float a[3]) {
+    for (int k = 0; k < n_slots; ++k)
+      for (int c = 0; c < 3; ++c) dm[k][c] *= a[c];
+    if (slot >= 0) {
+      const int k = slot_row(slot);
+      for (int c = 0; c < 3; ++c) dm[k][c] += m[c];  // (m a)' = m' a + m a'; m' a is applied above
+    }
+    for (int c = 0; c < 3; ++c) m[c] *= a[c];
+  }
+};
+
 struct Counters {
   uint64_t rays = 0, samples = 0, box_tests = 0, tri_tests = 0, prim_tests = 0, sphere_tests = 0, terrain_steps = 0,
            sky_evals = 0;
@@ -162,6 +422,7 @@ struct Scene {
   const df_sky* sky;
   const df_params* p;
   int prim_count;
+  const DiffMaps* diff = nullptr;  // differentiable mode, or null
 };
 
 struct Frame {
@@ -201,6 +462,8 @@ class Item {
   }
   int texture_color(int off) { return tex_color(off); }
   void set_record(const float* r) { std::memcpy(rec_, r, sizeof rec_); }
+  // forward-mode derivatives of the path tracer's samples (null: none)
+  void set_tangent(PathTangent* t) { tan_ = t; }
 
  private:
   const Scene& s_;
@@ -210,6 +473,10 @@ class Item {
   float rec_[R_SIZE];
   float ray_[6];
   float tmp_[3];
+  PathTangent* tan_ = nullptr;
+  // the last sun sample: its random numbers, cosine, and whether it lit the point
+  float sun_u_ = 0.0f, sun_v_ = 0.0f, sun_dp1_ = 0.0f;
+  bool sun_lit_ = false;
 
   const float* cam() const { return s_.a->camera; }
   const float* p3() const { return s_.a->point3fs; }
@@ -332,6 +599,9 @@ class Item {
   void do_sun(bool active, float px, float py, float pz, float nx, float ny, float nz, float ar, float ag, float ab);
   void render_ao();
   void render_pt(bool sky_active, bool sun_active);
+  void tan_sky(bool active, float dx, float dy, float dz);
+  void tan_vertex(int aslot, int eslot, const float a[3], const float e[3], bool sun_term, float pX, float pY,
+                  float pZ, float wX, float wY, float wZ);
   void render_rc(bool sky_active);
   void render_rm(bool sky_active);
   void render_rt(bool sky_active);
@@ -1190,6 +1460,7 @@ void Item::do_sky(bool active, float dx, float dy, float dz) {  // :2127-2236
 void Item::do_sun(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ, float aR, float aG,
                   float aB) {  // :2238-2347
   float r = 0.0f, g = 0.0f, b = 0.0f;
+  sun_lit_ = false;
   if (active) {
     const df_sky* S = s_.sky;
     const float sdX = S->sun_direction_world[0], sdY = S->sun_direction_world[1], sdZ = S->sun_direction_world[2];
@@ -1199,6 +1470,8 @@ void Item::do_sun(bool active, float pX, float pY, float pZ, float nX, float nY,
     const float to2 = toX * toX + toY * toY + toZ * toZ;
     const float u = next_float();
     const float v = next_float();
+    sun_u_ = u;
+    sun_v_ = v;
     const float sinMax2 = sunR2 / to2;
     const float cosMax = jsqrt(jmaxf(0.0f, 1.0f - sinMax2));
     const float cosT = u * (cosMax - 1.0f) + 1.0f;
@@ -1225,6 +1498,8 @@ void Item::do_sun(bool active, float pX, float pY, float pZ, float nX, float nY,
     if (dp0 > 0.0f && dp1 > 0.0f) {
       const float t = do_intersect_primitives(pX, pY, pZ, dX, dY, dZ, true);
       if (t >= INFINITY_ - 0.0001f) {
+        sun_lit_ = true;
+        sun_dp1_ = dp1;
         r = aR * 1.0f * dp1 * PI_RECIPROCAL;
         g = aG * 1.0f * dp1 * PI_RECIPROCAL;
         b = aB * 1.0f * dp1 * PI_RECIPROCAL;
@@ -1296,6 +1571,7 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
   float pr = 0.0f, pg = 0.0f, pb = 0.0f;
   float mr = 1.0f, mg = 1.0f, mb = 1.0f;
   const df_compiled_scene* A = s_.a;
+  if (tan_) tan_->begin();
   do {
     do_intersect_primitives(ox, oy, oz, dx, dy, dz, false);
     const float dist = rec_[R_T];
@@ -1303,6 +1579,7 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
     if (depth == 0) f_.primitive_offsets[gid_] = po;
     if (dist == INFINITY_ || po == -1) {
       do_sky(sky_active, dx, dy, dz);
+      if (tan_) tan_sky(sky_active, dx, dy, dz);
       pr += mr * tmp_[0] * PI_RECIPROCAL;
       pg += mg * tmp_[1] * PI_RECIPROCAL;
       pb += mb * tmp_[2] * PI_RECIPROCAL;
@@ -1323,14 +1600,29 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
     float aG = ((alb >> 8) & 0xFF) * COLOR_RECIPROCAL;
     float aB = (alb & 0xFF) * COLOR_RECIPROCAL;
     const int emi = tex_color(te);
-    const float eR = ((emi >> 16) & 0xFF) * COLOR_RECIPROCAL;
-    const float eG = ((emi >> 8) & 0xFF) * COLOR_RECIPROCAL;
-    const float eB = (emi & 0xFF) * COLOR_RECIPROCAL;
+    float eR = ((emi >> 16) & 0xFF) * COLOR_RECIPROCAL;
+    float eG = ((emi >> 8) & 0xFF) * COLOR_RECIPROCAL;
+    float eB = (emi & 0xFF) * COLOR_RECIPROCAL;
+    int aslot = -1, eslot = -1;
+    if (s_.diff && so / 6 < s_.diff->n_surfaces) {  // differentiable mode: parameters replace the textures
+      aslot = s_.diff->surface_albedo[so / 6];
+      eslot = s_.diff->surface_emission[so / 6];
+      if (aslot >= 0) {
+        aR = s_.diff->albedo[aslot * 3 + 0];
+        aG = s_.diff->albedo[aslot * 3 + 1];
+        aB = s_.diff->albedo[aslot * 3 + 2];
+      }
+      if (eslot >= 0) eR = eG = eB = s_.diff->emission[eslot];
+    }
     const int mat = jf2i(A->surfaces[so]);
     if (mat == 1 || mat == 3 || mat == 4) {
       do_sun(sun_active, pX, pY, pZ, wX, wY, wZ, aR, aG, aB);
     } else {
       tmp_[0] = tmp_[1] = tmp_[2] = 0.0f;
+    }
+    if (tan_) {
+      const float a3[3] = {aR, aG, aB}, e3[3] = {eR, eG, eB};
+      tan_vertex(aslot, eslot, a3, e3, mat == 1 || mat == 3 || mat == 4, pX, pY, pZ, wX, wY, wZ);
     }
     pr += mr * (eR + tmp_[0]);
     pg += mg * (eG + tmp_[1]);
@@ -1382,6 +1674,7 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
         mr *= m;
         mg *= m;
         mb *= m;
+        if (tan_) tan_->scale(m);
         if (isR) {
           ox = pX + wX * 0.02f;
           oy = pY + wY * 0.02f;
@@ -1415,6 +1708,10 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
           mr *= aR;
           mg *= aG;
           mb *= aB;
+          if (tan_) {
+            const float a3[3] = {aR, aG, aB};
+            tan_->albedo(aslot, a3);
+          }
         }
       }
     } else if (mat == 3) {  // Lambertian :3465-3515
@@ -1443,6 +1740,10 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
       mr *= aR;
       mg *= aG;
       mb *= aB;
+      if (tan_) {
+        const float a3[3] = {aR, aG, aB};
+        tan_->albedo(aslot, a3);
+      }
     } else if (mat == 4) {  // Phong :3516-3576
       const float exponent = 50.0f;
       const float u = next_float();
@@ -1474,6 +1775,10 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
       mr *= aR;
       mg *= aG;
       mb *= aB;
+      if (tan_) {
+        const float a3[3] = {aR, aG, aB};
+        tan_->albedo(aslot, a3);
+      }
     } else if (mat == 2) {  // Glass :3577-3670
       const float a = 1.5f - 1.0f, b = 1.5f + 1.0f, c = 1.0f / 1.5f, d = 1.5f / 1.0f;
       const float e = (a * a) / (b * b);
@@ -1507,6 +1812,7 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
         mr *= m;
         mg *= m;
         mb *= m;
+        if (tan_) tan_->scale(m);
         const float eps = isR ? 0.01f : 0.000001f;
         ox = pX + wX * eps;
         oy = pY + wY * eps;
@@ -1525,16 +1831,54 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
       mr *= aR;
       mg *= aG;
       mb *= aB;
+      if (tan_) {
+        const float a3[3] = {aR, aG, aB};
+        tan_->albedo(aslot, a3);
+      }
     }
   } while (depth < depth_max);
   const bool hit = do_intersect_primitives(ox, oy, oz, dx, dy, dz, true) < INFINITY_;
   if (!hit) {
     do_sky(sky_active, dx, dy, dz);
+    if (tan_) tan_sky(sky_active, dx, dy, dz);
     pr += mr * tmp_[0] * PI_RECIPROCAL;
     pg += mg * tmp_[1] * PI_RECIPROCAL;
     pb += mb * tmp_[2] * PI_RECIPROCAL;
   }
   film_add(pr, pg, pb);
+}
+
+// The sky term m * sky(d) / pi of a path that left the scene: through the
+// albedos in m, and through turbidity and elevation in the sky radiance.
+void Item::tan_sky(bool active, float dx, float dy, float dz) {
+  PathTangent& T = *tan_;
+  const double g[3] = {static_cast<double>(tmp_[0]) * PI_RECIPROCAL, static_cast<double>(tmp_[1]) * PI_RECIPROCAL,
+                       static_cast<double>(tmp_[2]) * PI_RECIPROCAL};
+  T.through_throughput(g);
+  D2 rgb[3];
+  if (!active || !dual_sky_rgb(*T.sky, dx, dy, dz, rgb)) return;
+  for (int c = 0; c < 3; ++c) {
+    T.dpix[T.i_turbidity * 3 + c] += T.m[c] * rgb[c].t * PI_RECIPROCAL;
+    T.dpix[T.i_elevation * 3 + c] += T.m[c] * rgb[c].e * PI_RECIPROCAL;
+  }
+}
+
+// The terms m * (e + sun) of a surface vertex: through the albedos in m, the
+// light's own emission parameter, and the sun term a * dp1 / pi through the
+// vertex's albedo and, at the sample's random numbers, the sun's elevation.
+void Item::tan_vertex(int aslot, int eslot, const float a[3], const float e[3], bool sun_term, float pX, float pY,
+                      float pZ, float wX, float wY, float wZ) {
+  PathTangent& T = *tan_;
+  const double g[3] = {static_cast<double>(e[0]) + tmp_[0], static_cast<double>(e[1]) + tmp_[1],
+                       static_cast<double>(e[2]) + tmp_[2]};
+  T.through_throughput(g);
+  if (eslot >= 0)
+    for (int c = 0; c < 3; ++c) T.dpix[(T.i_emission + eslot) * 3 + c] += T.m[c];
+  if (!sun_term || !sun_lit_) return;
+  if (aslot >= 0)
+    for (int c = 0; c < 3; ++c) T.dpix[(aslot * 3 + c) * 3 + c] += T.m[c] * sun_dp1_ * PI_RECIPROCAL;
+  const D2 dp1 = dual_sun_cosine(*T.sky, pX, pY, pZ, wX, wY, wZ, sun_u_, sun_v_);
+  for (int c = 0; c < 3; ++c) T.dpix[T.i_elevation * 3 + c] += T.m[c] * a[c] * dp1.e * PI_RECIPROCAL;
 }
 
 void Item::render_rc(bool sky_active) {  // :3707-3800
@@ -1863,6 +2207,135 @@ int oracle_render(const df_compiled_scene* scene, const df_sky* sky, const df_pa
     }
   }
   return 0;
+}
+
+// Differentiable mode of the path tracer (no counterpart in the reference):
+// the surface records mapped to continuous parameters (albedo group x RGB,
+// white emission; -1 keeps the texture), the parameter values, and the
+// sky's parameters (the df_sky rendered with is Sky.set of them).
+typedef struct oracle_diff {
+  int32_t n_albedo, n_emission;
+  const int32_t* surface_albedo;    // per surface record
+  const int32_t* surface_emission;  // per surface record
+  int64_t n_surfaces;
+  const float* albedo;    // n_albedo x 3
+  const float* emission;  // n_emission
+  double turbidity, elevation_deg, azimuth_deg;
+} oracle_diff;
+
+// L = sum over the crop [x0, x0 + crop_w) x [y0, y0 + crop_h) of
+// (film - target)^2, the film being the mean of `passes` path-traced samples
+// per pixel from the starting seeds given (width x height, not modified), and
+// its gradient dL/d[albedo groups x RGB, emission, turbidity, elevation per
+// degree] by forward-mode differentiation in double.  Needs the path tracer
+// with Russian roulette off, no wireframes and depth <= 64.  Writes the
+// crop's film (crop_w x crop_h x 3).  Returns 0 or DF_EINVAL.
+int oracle_loss_gradient(const df_compiled_scene* scene, const df_sky* sky, const df_params* params,
+                         const oracle_diff* diff, int32_t width, int32_t height, int32_t x0, int32_t y0,
+                         int32_t crop_w, int32_t crop_h, const int64_t* seeds, const float* target, int passes,
+                         int threads, float* film_out, double* loss_out, double* grad_out, int64_t n_grad) {
+  if (!scene || !sky || !params || !diff || !seeds || !target || !film_out || !loss_out || !grad_out) return DF_EINVAL;
+  if (params->renderer_type != DF_RENDERER_PATH_TRACER || params->wireframes == 1) return DF_EINVAL;
+  if (params->pt_depth_russian_roulette <= params->pt_depth_maximum) return DF_EINVAL;
+  if (params->pt_depth_maximum > PathTangent::kMaxSlots) return DF_EINVAL;
+  if (width <= 0 || height <= 0 || x0 < 0 || y0 < 0 || crop_w <= 0 || crop_h <= 0 || x0 + crop_w > width ||
+      y0 + crop_h > height || passes < 1)
+    return DF_EINVAL;
+  if (diff->n_albedo < 0 || diff->n_emission < 0 || diff->n_surfaces != static_cast<int64_t>(scene->surfaces_len / 6))
+    return DF_EINVAL;
+  const int n = 3 * diff->n_albedo + diff->n_emission + 2;
+  if (n_grad != n) return DF_EINVAL;
+  for (int64_t i = 0; i < diff->n_surfaces; ++i) {
+    if (diff->surface_albedo[i] < -1 || diff->surface_albedo[i] >= diff->n_albedo) return DF_EINVAL;
+    if (diff->surface_emission[i] < -1 || diff->surface_emission[i] >= diff->n_emission) return DF_EINVAL;
+  }
+  const DualSky K = dual_sky(sky, diff->turbidity, diff->elevation_deg, diff->azimuth_deg);
+  for (int i = 0; i < 3; ++i)  // the sky must be the one these parameters set up
+    if (std::fabs(K.world[i].v - sky->sun_direction_world[i]) > 1e-5) return DF_EINVAL;
+  if (std::fabs(diff->turbidity - sky->turbidity) > 1e-5 * diff->turbidity) return DF_EINVAL;
+
+  const DiffMaps maps{diff->n_albedo, diff->n_emission, diff->surface_albedo, diff->surface_emission,
+                      static_cast<int>(diff->n_surfaces), diff->albedo, diff->emission};
+  Scene s{scene, sky, params, static_cast<int>(scene->primitives_len / 3), &maps};
+  const size_t npix = static_cast<size_t>(width) * height;
+  std::vector<float> film(npix * 3, 0.0f);
+  std::vector<int32_t> film_samples(npix, 0), prim(npix, -1);
+  std::vector<uint8_t> bgra(npix * 4, 0);
+  std::vector<int64_t> sd(seeds, seeds + npix);
+  if (threads < 1) threads = 1;
+  if (threads > crop_h) threads = crop_h;
+  std::vector<std::vector<double>> grads(threads, std::vector<double>(n, 0.0));
+  std::vector<double> losses(threads, 0.0);
+  std::vector<Counters> cs(threads);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) {
+    pool.emplace_back([&, t]() {
+      std::vector<double> dpix(static_cast<size_t>(n) * 3);
+      PathTangent pt;
+      pt.sky = &K;
+      pt.n_albedo = diff->n_albedo;
+      pt.i_emission = 3 * diff->n_albedo;
+      pt.i_turbidity = n - 2;
+      pt.i_elevation = n - 1;
+      pt.dpix = dpix.data();
+      std::vector<double>& g = grads[t];
+      for (int y = y0 + crop_h * t / threads; y < y0 + crop_h * (t + 1) / threads; ++y) {
+        for (int x = x0; x < x0 + crop_w; ++x) {
+          const int gid = y * width + x;
+          std::fill(dpix.begin(), dpix.end(), 0.0);
+          for (int pass = 0; pass < passes; ++pass) {
+            Frame f{width, height, film.data(), film_samples.data(), nullptr, bgra.data(), prim.data(), sd.data(),
+                    pass == 0 ? 1 : 0};
+            Item it(s, f, gid, cs[t]);
+            it.set_tangent(&pt);
+            it.run();
+          }
+          float* fo = film_out + (static_cast<size_t>(y - y0) * crop_w + (x - x0)) * 3;
+          for (int c = 0; c < 3; ++c) {
+            const double dl = static_cast<double>(film[gid * 3 + c]) - target[gid * 3 + c];
+            fo[c] = film[gid * 3 + c];
+            losses[t] += dl * dl;
+            const double wgt = 2.0 * dl / passes;
+            for (int i = 0; i < n; ++i) g[i] += wgt * dpix[i * 3 + c];
+          }
+        }
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+  *loss_out = 0.0;
+  for (int i = 0; i < n; ++i) grad_out[i] = 0.0;
+  for (int t = 0; t < threads; ++t) {
+    *loss_out += losses[t];
+    for (int i = 0; i < n; ++i) grad_out[i] += grads[t][i];
+  }
+  return 0;
+}
+
+// The dual-number sky of the FP64 adjoint towards d: out[c] the radiance,
+// out[3 + c] its derivative with respect to turbidity, out[6 + c] with
+// respect to the sun elevation in degrees.  Returns 0 below the horizon.
+int oracle_sky_dual(const df_sky* frame, double turbidity, double elevation_deg, double azimuth_deg, const float d[3],
+                    double out[9]) {
+  const DualSky K = dual_sky(frame, turbidity, elevation_deg, azimuth_deg);
+  D2 rgb[3];
+  if (!dual_sky_rgb(K, d[0], d[1], d[2], rgb)) return 0;
+  for (int c = 0; c < 3; ++c) {
+    out[c] = rgb[c].v;
+    out[3 + c] = rgb[c].t;
+    out[6 + c] = rgb[c].e;
+  }
+  return 1;
+}
+
+// The dual-number sun cosine of the FP64 adjoint: out[0] the cosine, out[1]
+// its derivative with respect to the sun elevation in degrees.
+void oracle_sun_cosine_dual(const df_sky* frame, double turbidity, double elevation_deg, double azimuth_deg,
+                            const float p[3], const float n[3], float u, float v, double out[2]) {
+  const DualSky K = dual_sky(frame, turbidity, elevation_deg, azimuth_deg);
+  const D2 c = dual_sun_cosine(K, p[0], p[1], p[2], n[0], n[1], n[2], u, v);
+  out[0] = c.v;
+  out[1] = c.e;
 }
 
 // Single ray query against the scene (doIntersectPrimitives); writes the

@@ -135,3 +135,81 @@ def next_floats(seed, count):
     s = ctypes.c_int64(seed)
     out = np.array([lib().oracle_next_float(ctypes.byref(s)) for _ in range(count)], np.float32)
     return out, s.value
+
+
+class OracleDiff(ctypes.Structure):
+    _fields_ = [
+        ("n_albedo", ctypes.c_int32),
+        ("n_emission", ctypes.c_int32),
+        ("surface_albedo", ctypes.c_void_p),
+        ("surface_emission", ctypes.c_void_p),
+        ("n_surfaces", ctypes.c_int64),
+        ("albedo", ctypes.c_void_p),
+        ("emission", ctypes.c_void_p),
+        ("turbidity", ctypes.c_double),
+        ("elevation_deg", ctypes.c_double),
+        ("azimuth_deg", ctypes.c_double),
+    ]
+
+
+def loss_gradient(scene, params, sky, maps, albedo, emission, turbidity, elevation, azimuth, seeds, target, passes,
+                  crop, threads=None):
+    """FP64 forward-mode gradient of L = sum over the crop (x0, y0, w, h) of (film - target)^2, the film the mean
+    of `passes` path-traced samples from `seeds` (the full frame's, width x height from the scene's camera) with
+    the differentiable parameters.  `sky` is the sky rendered with (Sky.set of turbidity, elevation, azimuth).
+    Returns (loss, gradient [albedo groups x RGB, emission, turbidity, elevation per degree], crop film)."""
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    width, height = int(scene.camera[16]), int(scene.camera[17])
+    x0, y0, cw, ch = (int(v) for v in crop)
+    sa = np.ascontiguousarray(maps[0], np.int32)
+    se = np.ascontiguousarray(maps[1], np.int32)
+    al = np.ascontiguousarray(albedo, np.float32).reshape(-1)
+    em = np.ascontiguousarray(emission, np.float32).reshape(-1)
+    d = OracleDiff(al.size // 3, em.size, sa.ctypes.data, se.ctypes.data, sa.size, al.ctypes.data, em.ctypes.data,
+                   float(turbidity), float(elevation), float(azimuth))
+    sd = np.ascontiguousarray(seeds, np.int64)
+    tg = np.ascontiguousarray(target, np.float32).reshape(-1)
+    if sd.size != width * height or tg.size != width * height * 3:
+        raise ValueError("seeds / target do not match the camera's %d x %d" % (width, height))
+    n = 3 * (al.size // 3) + em.size + 2
+    grad = np.zeros(n, np.float64)
+    film = np.zeros(cw * ch * 3, np.float32)
+    loss = ctypes.c_double()
+    cs = scene.as_struct()
+    rc = lib().oracle_loss_gradient(ctypes.byref(cs), ctypes.byref(sky), ctypes.byref(params), ctypes.byref(d),
+                                    width, height, x0, y0, cw, ch, sd.ctypes.data_as(ctypes.c_void_p),
+                                    tg.ctypes.data_as(ctypes.c_void_p), int(passes), int(threads),
+                                    film.ctypes.data_as(ctypes.c_void_p), ctypes.byref(loss),
+                                    grad.ctypes.data_as(ctypes.c_void_p), ctypes.c_int64(n))
+    if rc != 0:
+        raise RuntimeError("oracle_loss_gradient failed: %d" % rc)
+    return loss.value, grad, film
+
+
+def sky_dual(sky, turbidity, elevation, azimuth, direction):
+    """The FP64 adjoint's sky towards `direction`: (rgb, d rgb / d turbidity, d rgb / d elevation) or None
+    below the horizon."""
+    d = (ctypes.c_float * 3)(*direction)
+    out = (ctypes.c_double * 9)()
+    f = lib().oracle_sky_dual
+    f.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                  ctypes.c_void_p]
+    if not f(ctypes.addressof(sky), turbidity, elevation, azimuth, ctypes.addressof(d), ctypes.addressof(out)):
+        return None
+    v = np.array(out[:], np.float64)
+    return v[0:3], v[3:6], v[6:9]
+
+
+def sun_cosine_dual(sky, turbidity, elevation, azimuth, point, normal, u, v):
+    """The FP64 adjoint's sun-sample cosine at random numbers (u, v): (cosine, d cosine / d elevation)."""
+    p = (ctypes.c_float * 3)(*point)
+    n = (ctypes.c_float * 3)(*normal)
+    out = (ctypes.c_double * 2)()
+    f = lib().oracle_sun_cosine_dual
+    f.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+    f.restype = None
+    f(ctypes.addressof(sky), turbidity, elevation, azimuth, ctypes.addressof(p), ctypes.addressof(n), u, v,
+      ctypes.addressof(out))
+    return out[0], out[1]
