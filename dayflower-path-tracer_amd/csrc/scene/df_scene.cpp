@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <stdexcept>
 #include <unordered_map>
 
 #include "df_scene_model.h"
@@ -224,14 +225,102 @@ Shape make_mesh(std::vector<Triangle> tris) {
 }
 
 int add_primitive(Scene& s, Shape shape, int surface, Vec3 translation) {
-  s.shapes.push_back(std::move(shape));
   Primitive p;
-  p.shape = static_cast<int>(s.shapes.size()) - 1;
   p.surface = surface;
   p.translation = translation;
+  return add_primitive(s, std::move(shape), p);
+}
+
+int add_primitive(Scene& s, Shape shape, const Primitive& transform_and_surface) {
+  s.shapes.push_back(std::move(shape));
+  Primitive p = transform_and_surface;
+  p.shape = static_cast<int>(s.shapes.size()) - 1;
   s.primitives.push_back(p);
   return static_cast<int>(s.primitives.size()) - 1;
 }
+
+// ---------------------------------------------------------------- transforms
+
+namespace {
+
+// Matrix44F.multiply (math4j Matrix44F.java): row-major, each element a
+// left-to-right sum of four products.
+void mat_mul(const float a[16], const float b[16], float out[16]) {
+  float r[16];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      r[i * 4 + j] = a[i * 4 + 0] * b[0 * 4 + j] + a[i * 4 + 1] * b[1 * 4 + j] + a[i * 4 + 2] * b[2 * 4 + j] +
+                     a[i * 4 + 3] * b[3 * 4 + j];
+  std::memcpy(out, r, sizeof r);
+}
+
+// Matrix44F.inverse: cofactors from the 2x2 minors of rows 1-2 and 3-4.
+bool mat_inverse(const float m[16], float out[16]) {
+  auto e = [&](int r, int c) { return m[(r - 1) * 4 + (c - 1)]; };
+  const float a = e(1, 1) * e(2, 2) - e(1, 2) * e(2, 1);
+  const float b = e(1, 1) * e(2, 3) - e(1, 3) * e(2, 1);
+  const float c = e(1, 1) * e(2, 4) - e(1, 4) * e(2, 1);
+  const float d = e(1, 2) * e(2, 3) - e(1, 3) * e(2, 2);
+  const float f0 = e(1, 2) * e(2, 4) - e(1, 4) * e(2, 2);
+  const float f = e(1, 3) * e(2, 4) - e(1, 4) * e(2, 3);
+  const float g = e(3, 1) * e(4, 2) - e(3, 2) * e(4, 1);
+  const float h = e(3, 1) * e(4, 3) - e(3, 3) * e(4, 1);
+  const float i = e(3, 1) * e(4, 4) - e(3, 4) * e(4, 1);
+  const float j = e(3, 2) * e(4, 3) - e(3, 3) * e(4, 2);
+  const float k = e(3, 2) * e(4, 4) - e(3, 4) * e(4, 2);
+  const float l = e(3, 3) * e(4, 4) - e(3, 4) * e(4, 3);
+  const float det = a * l - b * k + c * j + d * i - f0 * h + f * g;
+  if (std::fabs(det) < 1.0e-12f) return false;
+  const float r = 1.0f / det;
+  const float o[16] = {
+      (+e(2, 2) * l - e(2, 3) * k + e(2, 4) * j) * r, (-e(1, 2) * l + e(1, 3) * k - e(1, 4) * j) * r,
+      (+e(4, 2) * f - e(4, 3) * f0 + e(4, 4) * d) * r, (-e(3, 2) * f + e(3, 3) * f0 - e(3, 4) * d) * r,
+      (-e(2, 1) * l + e(2, 3) * i - e(2, 4) * h) * r, (+e(1, 1) * l - e(1, 3) * i + e(1, 4) * h) * r,
+      (-e(4, 1) * f + e(4, 3) * c - e(4, 4) * b) * r, (+e(3, 1) * f - e(3, 3) * c + e(3, 4) * b) * r,
+      (+e(2, 1) * k - e(2, 2) * i + e(2, 4) * g) * r, (-e(1, 1) * k + e(1, 2) * i - e(1, 4) * g) * r,
+      (+e(4, 1) * f0 - e(4, 2) * c + e(4, 4) * a) * r, (-e(3, 1) * f0 + e(3, 2) * c - e(3, 4) * a) * r,
+      (-e(2, 1) * j + e(2, 2) * h - e(2, 3) * g) * r, (+e(1, 1) * j - e(1, 2) * h + e(1, 3) * g) * r,
+      (-e(4, 1) * d + e(4, 2) * b - e(4, 3) * a) * r, (+e(3, 1) * d - e(3, 2) * b + e(3, 3) * a) * r};
+  std::memcpy(out, o, sizeof o);
+  return true;
+}
+
+}  // namespace
+
+bool transform_matrices(const Primitive& p, float o2w[16], float w2o[16]) {
+  const float qx = p.rotation[0], qy = p.rotation[1], qz = p.rotation[2], qw = p.rotation[3];
+  // Matrix44F.rotation(QuaternionF): rows u, v, w
+  const float wv[3] = {2.0f * (qx * qz - qw * qy), 2.0f * (qy * qz + qw * qx), 1.0f - 2.0f * (qx * qx + qy * qy)};
+  const float vv[3] = {2.0f * (qx * qy + qw * qz), 1.0f - 2.0f * (qx * qx + qz * qz), 2.0f * (qy * qz - qw * qx)};
+  const float uv[3] = {1.0f - 2.0f * (qy * qy + qz * qz), 2.0f * (qx * qy - qw * qz), 2.0f * (qx * qz + qw * qy)};
+  const float t[16] = {1, 0, 0, p.translation.x, 0, 1, 0, p.translation.y, 0, 0, 1, p.translation.z, 0, 0, 0, 1};
+  const float rot[16] = {uv[0], uv[1], uv[2], 0, vv[0], vv[1], vv[2], 0, wv[0], wv[1], wv[2], 0, 0, 0, 0, 1};
+  const float sc[16] = {p.scale.x, 0, 0, 0, 0, p.scale.y, 0, 0, 0, 0, p.scale.z, 0, 0, 0, 0, 1};
+  float tr[16];
+  mat_mul(t, rot, tr);
+  mat_mul(tr, sc, o2w);
+  return mat_inverse(o2w, w2o);
+}
+
+void transform_vertex(const float m[16], const float w[16], Vertex* v) {
+  // Point3F.transformAndDivide
+  const Vec3 q = v->position;
+  const float x = m[0] * q.x + m[1] * q.y + m[2] * q.z + m[3];
+  const float y = m[4] * q.x + m[5] * q.y + m[6] * q.z + m[7];
+  const float z = m[8] * q.x + m[9] * q.y + m[10] * q.z + m[11];
+  const float h = m[12] * q.x + m[13] * q.y + m[14] * q.z + m[15];
+  // MathF.equals(w, 1): Float.compare, so NaN never equals and -0 != +0 (irrelevant at 1)
+  v->position = h == 1.0f ? Vec3{x, y, z} : Vec3{x / h, y / h, z / h};
+  // Vector3F.transformTranspose
+  auto tt = [&](const Vec3& n) {
+    return Vec3{w[0] * n.x + w[4] * n.y + w[8] * n.z, w[1] * n.x + w[5] * n.y + w[9] * n.z,
+                w[2] * n.x + w[6] * n.y + w[10] * n.z};
+  };
+  v->normal = tt(v->normal);
+  v->tangent = tt(v->tangent);
+}
+
+float to_radians(float degrees) { return static_cast<float>(static_cast<double>(degrees) / 180.0 * M_PI); }
 
 // ---------------------------------------------------------------- camera
 
@@ -239,7 +328,6 @@ namespace {
 constexpr int kCamAperture = 0, kCamEye = 1, kCamFovX = 4, kCamFovY = 5, kCamFocal = 6, kCamU = 7, kCamV = 10,
               kCamW = 13, kCamResX = 16, kCamResY = 17, kCamLens = 18;
 
-inline float to_radians(float deg) { return static_cast<float>(static_cast<double>(deg) / 180.0 * M_PI); }
 inline float to_degrees(float rad) { return static_cast<float>(static_cast<double>(rad) * 180.0 / M_PI); }
 }  // namespace
 
@@ -583,9 +671,10 @@ std::vector<BvhNode> build_bvh(const std::vector<Triangle>& tris) {
 
 namespace {
 
-void push_matrix_translation(std::vector<float>& o2w, std::vector<float>& w2o, const Vec3& t) {
-  const float a[16] = {1, 0, 0, t.x, 0, 1, 0, t.y, 0, 0, 1, t.z, 0, 0, 0, 1};
-  const float b[16] = {1, 0, 0, -t.x, 0, 1, 0, -t.y, 0, 0, 1, -t.z, 0, 0, 0, 1};
+// objectToWorld / worldToObject rows of one primitive (SceneCompiler.java:695-747).
+void push_matrices(std::vector<float>& o2w, std::vector<float>& w2o, const Primitive& p) {
+  float a[16], b[16];
+  if (!transform_matrices(p, a, b)) throw std::invalid_argument("a primitive's transform cannot be inverted");
   o2w.insert(o2w.end(), a, a + 16);
   w2o.insert(w2o.end(), b, b + 16);
 }
@@ -752,7 +841,7 @@ Compiled compile(const Scene& s) {
     c.primitives.push_back(sh.type);
     c.primitives.push_back(shape_offset);
     c.primitives.push_back(prim.surface * 6);
-    push_matrix_translation(c.object_to_world, c.world_to_object, prim.translation);
+    push_matrices(c.object_to_world, c.world_to_object, prim);
   }
 
   c.point2fs = p2.take();

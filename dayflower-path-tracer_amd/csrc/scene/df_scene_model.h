@@ -98,7 +98,10 @@ struct Shape {
 struct Primitive {
   int shape = -1;    // index into Scene::shapes
   int surface = -1;  // index into Scene::surfaces
-  Vec3 translation;  // Transform position (rotation identity, scale 1)
+  // Transform (scene/Transform.java): position, rotation quaternion (x, y, z, w), scale
+  Vec3 translation;
+  float rotation[4] = {0.0f, 0.0f, 0.0f, 1.0f};
+  Vec3 scale{1.0f, 1.0f, 1.0f};
 };
 
 struct Scene {
@@ -148,6 +151,30 @@ Shape make_terrain(float frequency, float gain, float minimum, float maximum, in
 Shape make_triangle(const Triangle& t);
 Shape make_mesh(std::vector<Triangle> tris);
 int add_primitive(Scene& s, Shape shape, int surface, Vec3 translation = Vec3{});
+// As above with the full Transform of the primitive (p.shape is set here).
+int add_primitive(Scene& s, Shape shape, const Primitive& transform_and_surface);
+
+// Transform.doCalculateObjectToWorldAndWorldToObject (Transform.java): object
+// to world = translation * rotation(quaternion) * scaling, world to object its
+// inverse, row-major, with math4j Matrix44F's float arithmetic.  Returns
+// false where Matrix44F.inverse refuses (|determinant| < 1e-12).
+bool transform_matrices(const Primitive& p, float o2w[16], float w2o[16]);
+// Vertex.transformToWorldSpace (shape/Triangle.java:796-798): the position
+// by o2w (divided by w unless w is exactly 1), normal and tangent by the
+// transpose of w2o's upper 3x3.
+void transform_vertex(const float o2w[16], const float w2o[16], Vertex* v);
+// MathF.toRadians: (float) Math.toRadians(degrees)
+float to_radians(float degrees);
+
+// ---- Wavefront OBJ (scene/wavefront/ObjectLoader.java:118-212, 330-637) ----
+struct ObjMesh {
+  std::string group, material;
+  std::vector<Triangle> triangles;
+};
+// One mesh per run of triangles in the same group, in file order.  Returns
+// false with a message on I/O or syntax errors, out-of-range indices or a
+// file without faces.
+bool load_obj(const std::string& path, std::vector<ObjMesh>* out, std::string* error);
 
 // ---- camera (scene/Camera.java:239-264, 743-748, 877-946) ----
 void camera_default(float cam[19]);

@@ -1,10 +1,12 @@
 """MI355X renderer for Dayflower's RendererKernel hot path.
 
 ``scene``    - compiled scenes (flat CompiledScene arrays) and their builders.
+``assets``   - scene construction, Wavefront OBJ models and image textures.
 ``renderer`` - HIPRendererKernel, the renderer-kernel interface over the C ABI.
 """
 
 from . import _ffi  # noqa: F401
+from .assets import SceneBuilder, default_camera, image_rgb, material_showcase_scene, transform  # noqa: F401
 from .renderer import (  # noqa: F401
     HIPBatchRenderer,
     HIPError,

@@ -179,6 +179,13 @@ HIP_PROTOTYPES = {
     "df_batch_last_error": (ctypes.c_char_p, [_P]),
 }
 
+class Transform(ctypes.Structure):
+    """dfs_transform: Transform(position, rotation quaternion (x, y, z, w), scale)."""
+    _fields_ = [("position", ctypes.c_float * 3), ("rotation", ctypes.c_float * 4), ("scale", ctypes.c_float * 3)]
+
+
+SurfaceMapper = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_void_p)
+
 SCENE_PROTOTYPES = {
     "dfs_build_config": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(_P)]),
     "dfs_build_batch_scene": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(_P)]),
@@ -194,6 +201,37 @@ SCENE_PROTOTYPES = {
     "dfs_sky_set": (ctypes.c_int, [ctypes.POINTER(Sky), ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
     "dfs_camera_setup": (ctypes.c_int, [c_float_p] + [ctypes.c_float] * 11),
     "dfs_default_params": (None, [ctypes.POINTER(Params)]),
+    "dfs_camera_default": (ctypes.c_int, [c_float_p]),
+    "dfs_camera_set_eye": (ctypes.c_int, [c_float_p, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "dfs_camera_set_resolution": (ctypes.c_int, [c_float_p, ctypes.c_float, ctypes.c_float]),
+    "dfs_builder_create": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "dfs_builder_free": (None, [_P]),
+    "dfs_builder_last_error": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.c_size_t]),
+    "dfs_builder_constant_texture": (ctypes.c_int, [_P, c_float_p, c_int32_p]),
+    "dfs_builder_checkerboard_texture": (ctypes.c_int, [_P, c_float_p, c_float_p, ctypes.c_float, ctypes.c_float,
+                                                        ctypes.c_float, c_int32_p]),
+    "dfs_builder_fbm_texture": (ctypes.c_int, [_P, c_float_p, c_float_p, ctypes.c_float, ctypes.c_float,
+                                               ctypes.c_int32, c_int32_p]),
+    "dfs_builder_blend_texture": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_float, c_int32_p]),
+    "dfs_builder_bullseye_texture": (ctypes.c_int, [_P, c_float_p, c_float_p, c_int32_p]),
+    "dfs_builder_uv_texture": (ctypes.c_int, [_P, c_int32_p]),
+    "dfs_builder_surface_normal_texture": (ctypes.c_int, [_P, ctypes.c_int32, c_int32_p]),
+    "dfs_builder_image_texture": (ctypes.c_int, [_P, c_int32_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                                 ctypes.c_float, ctypes.c_float, c_int32_p]),
+    "dfs_builder_surface": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_float, ctypes.c_float, c_int32_p]),
+    "dfs_builder_plane": (ctypes.c_int, [_P, c_float_p, c_float_p, c_float_p, ctypes.c_int32,
+                                         ctypes.POINTER(Transform)]),
+    "dfs_builder_sphere": (ctypes.c_int, [_P, c_float_p, ctypes.c_float, ctypes.c_int32, ctypes.POINTER(Transform)]),
+    "dfs_builder_terrain": (ctypes.c_int, [_P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                           ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(Transform)]),
+    "dfs_builder_triangle": (ctypes.c_int, [_P, c_float_p, ctypes.c_int32, ctypes.POINTER(Transform)]),
+    "dfs_builder_mesh": (ctypes.c_int, [_P, c_float_p, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(Transform)]),
+    "dfs_builder_load_obj": (ctypes.c_int, [_P, ctypes.c_char_p, SurfaceMapper, ctypes.c_void_p, ctypes.c_int32,
+                                            ctypes.POINTER(Transform), c_int32_p]),
+    "dfs_builder_set_camera": (ctypes.c_int, [_P, c_float_p]),
+    "dfs_builder_set_sky": (ctypes.c_int, [_P, ctypes.POINTER(Sky)]),
+    "dfs_builder_compile": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
 }
 
 
