@@ -28,11 +28,25 @@ struct DPrim {
   float w2o[12];         // world -> object rows 1..3
   float o2w[12];         // object -> world rows 1..3
   float geo[8];          // sphere: pos.xyz, radius; plane: a.xyz, n.xyz; triangle: unused
-  // World-space box enclosing every point the primitive's test can return
-  // (enlarged to cover float rounding); bounded = 0 for planes and terrains.
-  float wb[6];           // min.xyz, max.xyz
-  int32_t bounded;
-  int32_t pad[3];
+  int32_t pad[10];
+};
+
+// World-space bounding sphere of what a primitive's test can hit, for
+// skipping primitives a ray cannot reach.  on = 0: never skipped (planes,
+// terrains, single triangles).  A ray skips the primitive only when the
+// sphere enlarged by R = r + 1e-4 |c - o| + k |c - o|^2 lies off its line,
+// wholly behind its origin or wholly beyond the closest hit so far.  k =
+// 1e-6 s / r, s <= 16 the squared stretch ratio of the primitive's
+// transform: the enlargement exceeds the float error of the object-space
+// sphere and box tests (about 1e-7 s |c - o|^2 / r and 1e-7 |c - o|) and of
+// this test by an order of magnitude, so a skipped primitive is one whose
+// own test would report no hit closer than the closest one.
+struct DCull {
+  float c[3];
+  float r;
+  float k;
+  int32_t on;
+  int32_t pad[2];
 };
 
 // Repacked BVH node (pre-order).  box = (min.xyz, max.xyz); next = index of
@@ -74,12 +88,19 @@ struct DScene {
   const DPrim* prims;
   const DNode* nodes;
   const DTri* tris;
+  const DCull* cull;
   // Terrain march positions tc_0 = 0.001, tc_{k+1} = tc_k + 0.01 * tc_k (float
   // arithmetic, every tc_k < 10): the sequence every terrain query visits.
   const float* march_tc;
   int32_t prim_count;
   int32_t march_n;
 };
+
+// Copies of the work counters per stage, one 64-byte line each: a wave adds
+// into the copy its global index selects and the host sums the copies.
+// Atomics on one line serialize (about 11 ns apiece), so a single copy made
+// the counter flush a large part of every launch.
+constexpr int kCounterSlots = 64;
 
 struct DCamera {
   float c[19];
@@ -193,19 +214,28 @@ struct DPaths {
   float4* sample;     // the pass's sample colour; w = 1: set the film instead of accumulating
   float4* ray0_o;     // primary ray, kept for the wireframe test
   float4* ray0_d;
-  int32_t* queue[2];  // live paths of even / odd bounces
+  int32_t* queue[2];  // live paths of even / odd bounces (kQueueStripes stripes of n entries)
   int32_t* shadow_queue;
-  int32_t* ctl;       // per-pass queue counts and work heads
-  int64_t n;          // slots
+  int32_t* ctl;       // per-pass queue counts and work heads (groups of kCtlGroup ints)
+  int64_t n;          // slots; also the capacity of one queue stripe
   int32_t ctl_len;
   int32_t pad;
 };
 
-// Length of DPaths::ctl for a maximum depth: B + 1 live-path counts, B + 1
-// shadow counts (the last stays zero) and 3B + 4 work heads, B = max(1, depth).
+// Work queues are striped: a wave appends to, and first drains, the stripe its
+// global index selects, so the atomics on a queue's count and work head spread
+// over kQueueStripes 64-byte lines (atomics on one line serialize).  A stripe
+// can receive every entry, so each holds n.
+constexpr int kQueueStripes = 16;
+constexpr int kLineInts = 16;                          // one 64-byte line
+constexpr int kCtlGroup = kQueueStripes * kLineInts;  // one count or head per stripe
+
+// Length of DPaths::ctl for a maximum depth, in ints: B + 1 live-path counts,
+// B + 1 shadow counts (the last stays zero) and 3B + 4 work heads, B =
+// max(1, depth), each a group of per-stripe lines.
 __host__ __device__ inline int wavefront_ctl_len(int depth_max) {
   const int b = depth_max > 1 ? depth_max : 1;
-  return 5 * b + 6;
+  return (5 * b + 6) * kCtlGroup;
 }
 
 }  // namespace dfk

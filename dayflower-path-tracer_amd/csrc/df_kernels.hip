@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K
   const int y = blockIdx.y * 16 + threadIdx.y;
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (x < P.width && y < P.height) render_pixel<M, R>(S, C, K, P, F, perm, x, y, cnt);
-  flush_counters(cnt, F.counters + kStageMega * kCounterCount);
+  flush_counters(cnt, F.counters, kStageMega);
 }
 
 // A batch of independent scenes in one persistent launch: each workgroup
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void batch_kernel(const DJob* __restrict__ job
     const int y = item.tile_y * 16 + threadIdx.y;
     unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (x < J.P.width && y < J.P.height) render_pixel<M, R>(J.S, J.C, J.K, J.P, J.F, perm, x, y, cnt);
-    flush_counters(cnt, J.F.counters + kStageMega * kCounterCount);
+    flush_counters(cnt, J.F.counters, kStageMega);
   }
 }
 

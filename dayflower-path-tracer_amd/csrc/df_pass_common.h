@@ -69,7 +69,8 @@ __device__ void image_pass(const DParams& P, const float film[3], int prim_off, 
 }
 
 constexpr int kCounterCount = 8;
-// DFrame::counters holds kCounterCount counters per stage (DF_STAGE_* order).
+// DFrame::counters holds, per stage (DF_STAGE_* order), kCounterSlots copies
+// of the kCounterCount counters (df_device_scene.h).
 enum Stage { kStageMega = 0, kStageRaygen, kStageTrace, kStageShade, kStageEscape, kStageFilm,
              kStageCount };
 
@@ -78,10 +79,15 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-// Adds each lane's counters into the frame's counters, one atomic per wave
-// and counter.  Called by every lane of the wave.
+// Adds each lane's counters into the frame's counters of `stage`, one atomic
+// per wave and non-zero counter, into the wave's copy.  Called by every lane
+// of the wave.
 __device__ __forceinline__ void flush_counters(const unsigned long long cnt[kCounterCount],
-                                               unsigned long long* out) {
+                                               unsigned long long* counters, int stage) {
+  const unsigned block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const unsigned waves_per_block = (blockDim.x * blockDim.y * blockDim.z + 63) / 64;
+  const unsigned wave = block * waves_per_block + (threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z)) / 64;
+  unsigned long long* out = counters + (stage * kCounterSlots + wave % kCounterSlots) * kCounterCount;
   const bool leader = __lane_id() == 0;
   for (int i = 0; i < kCounterCount; ++i) {
     const unsigned long long v = wave_sum(cnt[i]);

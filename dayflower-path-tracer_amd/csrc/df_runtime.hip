@@ -22,6 +22,7 @@
 
 using dfk::DBatchItem;
 using dfk::DCamera;
+using dfk::DCull;
 using dfk::DFrame;
 using dfk::DJob;
 using dfk::DNode;
@@ -102,7 +103,8 @@ struct df_ctx {
 
 namespace {
 
-constexpr size_t kCounterBytes = sizeof(df_counters) * DF_STAGE_COUNT;
+constexpr size_t kCounterBytes = sizeof(df_counters) * DF_STAGE_COUNT * dfk::kCounterSlots;
+
 
 int fail(df_ctx* c, int code, const char* fmt, ...) {
   char buf[512];
@@ -323,48 +325,42 @@ int repack_mesh(df_ctx* c, const HostScene& h, int32_t base, std::vector<DNode>&
 // mapped with the inverse of world_to_object computed here in double (the
 // kernel transforms rays with world_to_object), and enlarged so that the
 // float rounding of both tests stays inside it.
-void world_bounds(const HostScene& h, const std::vector<DNode>& nodes, DPrim& p) {
-  p.bounded = 0;
-  double lo[3], hi[3];
-  switch (p.type) {
-    case 2:  // sphere: centre +- radius
-      for (int k = 0; k < 3; ++k) {
-        const double r = std::fabs(static_cast<double>(p.geo[3]));
-        lo[k] = p.geo[k] - r;
-        hi[k] = p.geo[k] + r;
-      }
-      break;
-    case 4: {  // triangle: its vertices
-      const int32_t* t = h.triangles.data() + p.shape_offset;
-      for (int k = 0; k < 3; ++k) lo[k] = hi[k] = h.point3fs[t[0] + k];
-      for (int v = 1; v < 3; ++v)
-        for (int k = 0; k < 3; ++k) {
-          lo[k] = std::min(lo[k], static_cast<double>(h.point3fs[t[v] + k]));
-          hi[k] = std::max(hi[k], static_cast<double>(h.point3fs[t[v] + k]));
-        }
-      break;
+// The primitive's world-space bounding sphere (DCull): the object-space
+// sphere of a sphere primitive, or the sphere around a mesh's BVH root box
+// (a ray that misses the root box reaches no triangle), mapped to the world
+// by the inverse of world_to_object with the radius scaled by the map's
+// largest stretch.  Primitives under a strongly anisotropic map (stretch
+// ratio above 4) are never skipped: the enlargement argument assumes a
+// near-conformal map.
+DCull cull_bounds(const HostScene& h, const std::vector<DNode>& nodes, const DPrim& p) {
+  (void)h;
+  DCull out{};
+  double c[3], r;
+  if (p.type == 2) {
+    for (int k = 0; k < 3; ++k) c[k] = p.geo[k];
+    r = std::fabs(static_cast<double>(p.geo[3]));
+  } else if (p.type == 5) {
+    const DNode& n = nodes[p.node_root];
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      c[k] = 0.5 * (static_cast<double>(n.mn[k]) + n.mx[k]);
+      const double e = static_cast<double>(n.mx[k]) - n.mn[k];
+      d2 += e * e;
     }
-    case 5: {  // mesh: the root box (a ray missing it reaches no triangle)
-      const DNode& r = nodes[p.node_root];
-      for (int k = 0; k < 3; ++k) {
-        lo[k] = r.mn[k];
-        hi[k] = r.mx[k];
-      }
-      break;
-    }
-    default:
-      return;
+    r = 0.5 * std::sqrt(d2);
+  } else {
+    return out;
   }
   for (int k = 0; k < 3; ++k)
-    if (!std::isfinite(lo[k]) || !std::isfinite(hi[k]) || lo[k] > hi[k]) return;
-  // inverse of the affine world->object map [A | b]: x = A^-1 (y - b)
+    if (!std::isfinite(c[k])) return out;
+  if (!std::isfinite(r) || !(r > 0.0)) return out;
   const float* m = p.w2o;
   const double a[3][3] = {{m[0], m[1], m[2]}, {m[4], m[5], m[6]}, {m[8], m[9], m[10]}};
   const double b[3] = {m[3], m[7], m[11]};
   const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
                      a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
                      a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
-  if (!std::isfinite(det) || std::fabs(det) < 1e-12) return;
+  if (!std::isfinite(det) || std::fabs(det) < 1e-12) return out;
   double inv[3][3];
   inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
   inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
@@ -375,33 +371,59 @@ void world_bounds(const HostScene& h, const std::vector<DNode>& nodes, DPrim& p)
   inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
   inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
   inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
-  double wlo[3] = {1e300, 1e300, 1e300}, whi[3] = {-1e300, -1e300, -1e300};
-  for (int corner = 0; corner < 8; ++corner) {
-    const double y[3] = {(corner & 1) ? hi[0] : lo[0], (corner & 2) ? hi[1] : lo[1], (corner & 4) ? hi[2] : lo[2]};
-    for (int r = 0; r < 3; ++r) {
-      const double x = inv[r][0] * (y[0] - b[0]) + inv[r][1] * (y[1] - b[1]) + inv[r][2] * (y[2] - b[2]);
-      wlo[r] = std::min(wlo[r], x);
-      whi[r] = std::max(whi[r], x);
-    }
+  // extreme eigenvalues of M = inv^T inv (closed form for symmetric 3x3)
+  double M[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) M[i][j] = inv[0][i] * inv[0][j] + inv[1][i] * inv[1][j] + inv[2][i] * inv[2][j];
+  const double p1 = M[0][1] * M[0][1] + M[0][2] * M[0][2] + M[1][2] * M[1][2];
+  const double q = (M[0][0] + M[1][1] + M[2][2]) / 3.0;
+  double e_max, e_min;
+  if (p1 < 1e-30 * q * q) {
+    e_max = std::max({M[0][0], M[1][1], M[2][2]});
+    e_min = std::min({M[0][0], M[1][1], M[2][2]});
+  } else {
+    const double p2 = (M[0][0] - q) * (M[0][0] - q) + (M[1][1] - q) * (M[1][1] - q) +
+                      (M[2][2] - q) * (M[2][2] - q) + 2.0 * p1;
+    const double pp = std::sqrt(p2 / 6.0);
+    double B[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) B[i][j] = (M[i][j] - (i == j ? q : 0.0)) / pp;
+    const double detB = B[0][0] * (B[1][1] * B[2][2] - B[1][2] * B[2][1]) -
+                        B[0][1] * (B[1][0] * B[2][2] - B[1][2] * B[2][0]) +
+                        B[0][2] * (B[1][0] * B[2][1] - B[1][1] * B[2][0]);
+    const double phi = std::acos(std::max(-1.0, std::min(1.0, detB / 2.0))) / 3.0;
+    e_max = q + 2.0 * pp * std::cos(phi);
+    e_min = q + 2.0 * pp * std::cos(phi + 2.0 * M_PI / 3.0);
   }
-  double scale = 1.0;
-  for (int k = 0; k < 3; ++k) scale = std::max(scale, std::max(std::fabs(wlo[k]), std::fabs(whi[k])));
-  const double pad = 1e-4 * scale + 1e-3;
+  if (!(e_min > 0.0) || e_max > 16.0 * e_min) return out;  // stretch ratio above 4
+  const double stretch = std::sqrt(e_max) * (1.0 + 1e-6);
+  double cw[3];
+  for (int k = 0; k < 3; ++k)
+    cw[k] = inv[k][0] * (c[0] - b[0]) + inv[k][1] * (c[1] - b[1]) + inv[k][2] * (c[2] - b[2]);
+  const double rw = r * stretch * 1.0001;
   for (int k = 0; k < 3; ++k) {
-    if (!std::isfinite(wlo[k]) || !std::isfinite(whi[k])) return;
-    p.wb[k] = static_cast<float>(wlo[k] - pad);
-    p.wb[3 + k] = static_cast<float>(whi[k] + pad);
+    if (!std::isfinite(cw[k])) return out;
+    out.c[k] = static_cast<float>(cw[k]);
   }
-  p.bounded = 1;
+  // the float centre is off by at most half an ulp of each coordinate: fold that into the radius
+  double ulp = 0.0;
+  for (int k = 0; k < 3; ++k) ulp += std::fabs(cw[k] - out.c[k]);
+  out.r = std::nextafter(static_cast<float>(rw + ulp), INFINITY);
+  // the sphere test's float error in world units is below 1e-7 (e_max / e_min) |c - o|^2 / rw
+  out.k = std::nextafter(static_cast<float>(1e-6 * (e_max / e_min) / rw), INFINITY);
+  if (!std::isfinite(out.r) || !std::isfinite(out.k)) return DCull{};
+  out.on = 1;
+  return out;
 }
 
 int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::vector<DNode>& nodes,
-                std::vector<DTri>& tris) {
+                std::vector<DTri>& tris, std::vector<DCull>& cull) {
   const size_t np = h.primitives.size() / 3;
   if (h.primitives.size() % 3 != 0 || np == 0) return fail(c, DF_EINVAL, "primitives length %zu", h.primitives.size());
   if (h.o2w.size() < np * 16 || h.w2o.size() < np * 16)
     return fail(c, DF_EINVAL, "transform arrays hold fewer than 16 floats per primitive");
   prims.assign(np, DPrim{});
+  cull.assign((np + 7) / 8 * 8, DCull{});  // padded: the skip pass reads the spheres eight at a time
   for (size_t i = 0; i < np; ++i) {
     DPrim& p = prims[i];
     p.type = h.primitives[i * 3 + 0];
@@ -454,7 +476,7 @@ int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::v
       default:
         break;  // unknown shapes are never hit (the reference skips them)
     }
-    world_bounds(h, nodes, p);
+    cull[i] = cull_bounds(h, nodes, p);
   }
   return DF_OK;
 }
@@ -598,12 +620,13 @@ int rebuild_device_scene(df_ctx* c) {
   std::vector<DPrim> prims;
   std::vector<DNode> nodes;
   std::vector<DTri> tris;
-  int rc = build_prims(c, c->hs, prims, nodes, tris);
+  std::vector<DCull> cull;
+  int rc = build_prims(c, c->hs, prims, nodes, tris, cull);
   if (rc) return rc;
   // buffers: 0 point2fs 1 point3fs 2 vector3fs 3 planes 4 spheres 5 terrains
   // 6 triangles 7 textures 8 surfaces 9 primitives 10 prims 11 nodes 12 tris
-  // 13 terrain march positions
-  c->scene_bufs.resize(14);
+  // 13 terrain march positions 14 cull spheres
+  c->scene_bufs.resize(15);
   const std::vector<float> march = terrain_march_positions();
   const HostScene& h = c->hs;
   if ((rc = upload(c, c->scene_bufs[0], h.point2fs)) || (rc = upload(c, c->scene_bufs[1], h.point3fs)) ||
@@ -612,7 +635,8 @@ int rebuild_device_scene(df_ctx* c) {
       (rc = upload(c, c->scene_bufs[6], h.triangles)) || (rc = upload(c, c->scene_bufs[7], h.textures)) ||
       (rc = upload(c, c->scene_bufs[8], h.surfaces)) || (rc = upload(c, c->scene_bufs[9], h.primitives)) ||
       (rc = upload(c, c->scene_bufs[10], prims)) || (rc = upload(c, c->scene_bufs[11], nodes)) ||
-      (rc = upload(c, c->scene_bufs[12], tris)) || (rc = upload(c, c->scene_bufs[13], march)))
+      (rc = upload(c, c->scene_bufs[12], tris)) || (rc = upload(c, c->scene_bufs[13], march)) ||
+      (rc = upload(c, c->scene_bufs[14], cull)))
     return rc;
   DScene& d = c->ds;
   d.point2fs = static_cast<const float*>(c->scene_bufs[0].p);
@@ -629,6 +653,7 @@ int rebuild_device_scene(df_ctx* c) {
   d.nodes = static_cast<const DNode*>(c->scene_bufs[11].p);
   d.tris = static_cast<const DTri*>(c->scene_bufs[12].p);
   d.march_tc = static_cast<const float*>(c->scene_bufs[13].p);
+  d.cull = static_cast<const DCull*>(c->scene_bufs[14].p);
   d.prim_count = static_cast<int32_t>(prims.size());
   d.march_n = static_cast<int32_t>(march.size());
   return DF_OK;
@@ -661,8 +686,8 @@ int ensure_paths(df_ctx* c, int depth_max) {
   for (DevBuf& b : c->path_f4)
     if ((rc = alloc_buf(c, b, n * sizeof(float4)))) return rc;
   for (DevBuf& b : c->path_q)
-    if ((rc = alloc_buf(c, b, n * sizeof(int32_t)))) return rc;
-  const int ctl_alloc = ctl_len > 2048 ? ctl_len : 2048;
+    if ((rc = alloc_buf(c, b, n * dfk::kQueueStripes * sizeof(int32_t)))) return rc;
+  const int ctl_alloc = ctl_len > 64 * dfk::kCtlGroup ? ctl_len : 64 * dfk::kCtlGroup;
   if ((rc = alloc_buf(c, c->path_ctl, static_cast<size_t>(ctl_alloc) * sizeof(int32_t)))) return rc;
   dfk::DPaths& w = c->paths;
   float4** f4[11] = {&w.ray_o, &w.ray_d, &w.thru, &w.rad, &w.hit, &w.sh_o, &w.sh_d, &w.rad_lit, &w.sample,
@@ -853,7 +878,8 @@ int df_validate_scene(const df_compiled_scene* s, char* msg, size_t cap) {
     std::vector<DPrim> prims;
     std::vector<DNode> nodes;
     std::vector<DTri> tris;
-    rc = build_prims(&scratch, h, prims, nodes, tris);
+    std::vector<DCull> cull;
+    rc = build_prims(&scratch, h, prims, nodes, tris, cull);
   }
   if (msg && cap) {
     std::snprintf(msg, cap, "%s", rc ? scratch.err.c_str() : "ok");
@@ -1211,11 +1237,25 @@ int df_set_film(df_ctx* c, const float* rgb, size_t rgb_n, const int32_t* sample
   return DF_OK;
 }
 
+// Sums the per-wave copies of the device counters: out[stage][counter].
+static int read_counters(df_ctx* c, uint64_t out[DF_STAGE_COUNT][8]) {
+  std::vector<uint64_t> raw(kCounterBytes / sizeof(uint64_t));
+  DF_HIP(c, hipMemcpyAsync(raw.data(), c->counters.p, kCounterBytes, hipMemcpyDeviceToHost, c->stream));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  for (int st = 0; st < DF_STAGE_COUNT; ++st)
+    for (int i = 0; i < 8; ++i) {
+      uint64_t v = 0;
+      for (int s = 0; s < dfk::kCounterSlots; ++s) v += raw[(static_cast<size_t>(st) * dfk::kCounterSlots + s) * 8 + i];
+      out[st][i] = v;
+    }
+  return DF_OK;
+}
+
 int df_get_counters(df_ctx* c, df_counters* out) {
   if (!c || !out) return DF_EINVAL;
   DF_HIP(c, hipStreamSynchronize(c->stream));
   uint64_t all[DF_STAGE_COUNT][8];
-  DF_HIP(c, hipMemcpy(all, c->counters.p, kCounterBytes, hipMemcpyDeviceToHost));
+  if (const int rc = read_counters(c, all)) return rc;
   uint64_t sum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int st = 0; st < DF_STAGE_COUNT; ++st)
     for (int i = 0; i < 8; ++i) sum[i] += all[st][i];
@@ -1253,7 +1293,7 @@ int df_get_stage_stats(df_ctx* c, int32_t stage, df_counters* counters, double* 
   if (rc) return rc;
   if (counters) {
     uint64_t all[DF_STAGE_COUNT][8];
-    DF_HIP(c, hipMemcpy(all, c->counters.p, kCounterBytes, hipMemcpyDeviceToHost));
+    if (const int r = read_counters(c, all)) return r;
     std::memcpy(counters, all[stage], sizeof(df_counters));
   }
   if (ms) *ms = c->stage_ms[stage];

@@ -146,8 +146,38 @@ struct Tracer {
     cst = -1;
     cso = -1;
     const int n = S.prim_count;
-    for (int i = 0; i < n; ++i) {
-      const DPrim pr = load_uniform(S.prims, i);  // wave-uniform: scalar loads
+    // Primitives in blocks of 64: first a branch-free pass over the block's
+    // bounding spheres marks, per lane, the primitives the ray can reach
+    // (approximate reciprocal and square root: the spheres' enlargement
+    // covers their error), then the marked primitives run in index order.
+    const float dd = dx * dx + dy * dy + dz * dz;
+    const float ia = __builtin_amdgcn_rcpf(dd), dl = __builtin_amdgcn_sqrtf(dd);
+    for (int base = 0; base < n; base += 64) {
+      const int m = n - base < 64 ? n - base : 64;
+      uint64_t cand = 0, any = 0;  // this lane's marks; the wave's union
+      for (int j0 = 0; j0 < m; j0 += 8) {  // the sphere array is padded to a multiple of 8
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int j = j0 + u;
+          const DCull cu = load_uniform(S.cull, base + j);  // wave-uniform: scalar loads
+          const float lx = cu.c[0] - ox, ly = cu.c[1] - oy, lz = cu.c[2] - oz;
+          const float l2 = lx * lx + ly * ly + lz * lz;
+          const float tca = lx * dx + ly * dy + lz * dz;
+          const float R = cu.r + 1.0e-4f * __builtin_amdgcn_sqrtf(l2) + cu.k * l2;
+          const float d2 = l2 - tca * tca * ia;
+          // comparisons are false for NaN: a ray with non-finite terms is never skipped
+          const int miss = static_cast<int>(d2 > R * R) | static_cast<int>(tca < -R * dl);
+          const int keep = static_cast<int>(j < m) & ((cu.on == 0) | (miss ^ 1));
+          cand |= static_cast<uint64_t>(keep) << j;
+          any |= __ballot(keep) != 0 ? uint64_t{1} << j : uint64_t{0};
+        }
+      }
+      while (any) {
+      const int j = __builtin_ctzll(any);
+      any &= any - 1;
+      if (!((cand >> j) & 1)) continue;
+      const int i = base + j;
+      const DPrim pr = load_uniform(S.prims, i);
       const int st = pr.type;
       ++prim_tests;
       const float* m = pr.w2o;
@@ -215,6 +245,7 @@ struct Tracer {
         cso = pr.shape_offset;
       }
       if (testing && ci != -1) return closest;
+      }
     }
     return closest;
   }

@@ -40,32 +40,54 @@ __device__ __forceinline__ void stage_perm(int* perm) {
   __syncthreads();
 }
 
-// Appends `value` for the lanes with `pred` set: one atomic per wave, the
-// lanes' entries in lane order.  Called by every active lane of the wave.
-__device__ __forceinline__ void wave_append(int32_t* q, int32_t* count, bool pred, int32_t value) {
+// Global index of the calling wave (any block shape).
+__device__ __forceinline__ int global_wave() {
+  const int block = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int per_block = (blockDim.x * blockDim.y * blockDim.z + 63) / 64;
+  return block * per_block + (threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z)) / 64;
+}
+
+// Appends `value` for the lanes with `pred` set to the striped queue q (count
+// group `counts`, stripe capacity `cap`): one atomic per wave, on the count of
+// the wave's stripe, the lanes' entries in lane order.  Called by every active
+// lane of the wave.
+__device__ __forceinline__ void wave_append(int32_t* q, int64_t cap, int32_t* counts, bool pred, int32_t value) {
   const unsigned long long mask = __ballot(pred);
   if (mask == 0ull) return;
   const int lane = __lane_id();
   const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+  const int stripe = global_wave() % kQueueStripes;
   int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  if (lane == leader) base = atomicAdd(counts + stripe * kLineInts, __popcll(mask));
   base = __shfl(base, leader, 64);
-  if (pred) q[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+  if (pred) q[stripe * cap + base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
-// Runs body(valid, entry) over the queue, 64 entries per wave at a time.
-// Every lane of a wave takes part in every call (valid is false past the end).
+// Runs body(valid, entry) over the striped queue q (count group `counts`,
+// head group `heads`), 64 entries per wave at a time: a wave drains its own
+// stripe first, then helps with the others (skipping, without an atomic, a
+// stripe whose head has passed its end).  Every lane of a wave takes part in
+// every call (valid is false past the end).
 template <class Body>
-__device__ __forceinline__ void drain(const int32_t* q, const int32_t* count, int32_t* head, Body&& body) {
-  const int n = *count;
-  for (;;) {
-    int base = 0;
-    if (__lane_id() == 0) base = atomicAdd(head, 64);
-    base = __shfl(base, 0, 64);
-    if (base >= n) break;
-    const int k = base + __lane_id();
-    const bool valid = k < n;
-    body(valid, valid ? q[k] : 0);
+__device__ __forceinline__ void drain(const int32_t* q, int64_t cap, const int32_t* counts, int32_t* heads,
+                                      Body&& body) {
+  const int first = global_wave() % kQueueStripes;
+  for (int t = 0; t < kQueueStripes; ++t) {
+    const int stripe = (first + t) % kQueueStripes;
+    const int n = counts[stripe * kLineInts];
+    const int chunks = (n + 63) / 64;
+    int32_t* head = heads + stripe * kLineInts;
+    if (t > 0 && __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= chunks) continue;
+    const int32_t* qs = q + stripe * cap;
+    for (;;) {
+      int ch = 0;
+      if (__lane_id() == 0) ch = atomicAdd(head, 1);
+      ch = __shfl(ch, 0, 64);
+      if (ch >= chunks) break;
+      const int k = ch * 64 + __lane_id();
+      const bool valid = k < n;
+      body(valid, valid ? qs[k] : 0);
+    }
   }
 }
 
@@ -102,11 +124,13 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
       W.sample[gid] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);  // filmSetColor(0)
     }
   }
-  wave_append(W.queue[0], W.ctl, valid, gid);
+  wave_append(W.queue[0], W.n, W.ctl, valid, gid);
 }
 
+// Occupancy hint: four waves per SIMD (128 VGPRs, a few spills) measured
+// faster than the three an unconstrained allocation allows.
 template <int M>
-__global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+__global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
                                                      int bounce) {
@@ -115,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K
   Tracer<M> tr(S, C, K, P, perm);
   const bool sky_active = K.sky_active == 1;
   const bool sun_active = K.sun_active == 1;
-  drain(q, count, head, [&](bool valid, int32_t i) {
+  drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     bool more = false, shadow = false;
     if (valid) {
       const float4 o4 = W.ray_o[i], d4 = W.ray_d[i], hv = W.hit[i], m4 = W.thru[i], p4 = W.rad[i];
@@ -142,12 +166,12 @@ __global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K
         W.sample[i] = f4(p, 0.0f);
       }
     }
-    wave_append(q_next, count_next, valid && more, i);
-    wave_append(W.shadow_queue, count_shadow, valid && shadow, more ? i : (i | kFinish));
+    wave_append(q_next, W.n, count_next, valid && more, i);
+    wave_append(W.shadow_queue, W.n, count_shadow, valid && shadow, more ? i : (i | kFinish));
   });
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   take_counters(tr, cnt);
-  flush_counters(cnt, F.counters + kStageShade * kCounterCount);
+  flush_counters(cnt, F.counters, kStageShade);
 }
 
 // The ray queries of one bounce in one launch: first the sun's shadow
@@ -160,13 +184,15 @@ __global__ __launch_bounds__(kThreads) void wf_shade(DScene S, DCamera C, DSky K
 // path data: a shadow query writes the radiance and the sample, a
 // closest-hit query the hit record; an escape test reads the radiance, so
 // the last sun queries run in a launch before it.
+// Occupancy hint: six waves per SIMD (80 VGPRs, a few spills); the BVH walks
+// are latency-bound, and six measured faster than five or eight.
 template <int M, bool kEscape>
-__global__ __launch_bounds__(kThreads) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+__global__ __launch_bounds__(kThreads, 6) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce) {
   Tracer<M> tr(S, C, K, P, kPerm);
-  drain(W.shadow_queue, shadow_count, shadow_head, [&](bool valid, int32_t e) {
+  drain(W.shadow_queue, W.n, shadow_count, shadow_head, [&](bool valid, int32_t e) {
     if (!valid) return;
     const int32_t i = e & (kFinish - 1);
     const float4 o = W.sh_o[i], d = W.sh_d[i];
@@ -183,7 +209,7 @@ __global__ __launch_bounds__(kThreads) void wf_trace(DScene S, DCamera C, DSky K
     }
   });
   const bool sky_active = K.sky_active == 1;
-  drain(q, count, head, [&](bool valid, int32_t i) {
+  drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     if (!valid) return;
     const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
     if constexpr (kEscape) {
@@ -201,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void wf_trace(DScene S, DCamera C, DSky K
   });
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   take_counters(tr, cnt);
-  flush_counters(cnt, F.counters + (kEscape ? kStageEscape : kStageTrace) * kCounterCount);
+  flush_counters(cnt, F.counters, kEscape ? kStageEscape : kStageTrace);
 }
 
 // filmAddColor / filmSetColor, the wireframe overlay and, after the last pass,
@@ -250,7 +276,7 @@ __global__ __launch_bounds__(256) void wf_film(DScene S, DCamera C, DSky K, DPar
     if (last) image_pass<M>(P, film, F.primitive_offsets[gid], F.image_float + gid * 3, F.image_bgra + gid * 4);
     take_counters(tr, cnt);
   }
-  flush_counters(cnt, F.counters + kStageFilm * kCounterCount);
+  flush_counters(cnt, F.counters, kStageFilm);
 }
 
 // Resident workgroups of a persistent kernel: occupancy x compute units.
@@ -273,10 +299,13 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
   // ctl: [0, B] live-path counts per bounce (B = escape queue), [B+1, 2B]
-  // shadow counts (plus one that stays zero), then the work heads.
+  // shadow counts (plus one that stays zero), then the work heads; each
+  // count and head a group of kCtlGroup ints (one line per queue stripe).
   int32_t* ctl = w.ctl;
-  int32_t* shadow_count = ctl + bounces + 1;
-  int32_t* heads = ctl + 2 * bounces + 2;
+  auto group = [&](int g) { return ctl + static_cast<size_t>(g) * kCtlGroup; };
+  auto live = [&](int b) { return group(b); };
+  auto shadow_count = [&](int b) { return group(bounces + 1 + b); };
+  auto head = [&](int h) { return group(2 * bounces + 2 + h); };
   const dim3 pix_block(16, 16);
   const dim3 pix_grid((p.width + 15) / 16, (p.height + 15) / 16);
   auto mark = [&](int stage, int begin) {
@@ -289,14 +318,14 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
     mark(kStageRaygen, 0);
     for (int b = 0; b < bounces; ++b) {
       // shadow queries of bounce b - 1 (none before the first bounce: that count stays 0)
-      int32_t* sc = b > 0 ? shadow_count + (b - 1) : shadow_count + bounces;
+      int32_t* sc = b > 0 ? shadow_count(b - 1) : shadow_count(bounces);
       mark(kStageTrace, 1);
       hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
-                         heads + 3 * b, w.queue[b & 1], ctl + b, heads + 3 * b + 1, b == 0 ? 1 : 0);
+                         head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0);
       mark(kStageTrace, 0);
       mark(kStageShade, 1);
       hipLaunchKernelGGL(wf_shade<M>, dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
-                         ctl + b, heads + 3 * b + 2, w.queue[(b + 1) & 1], ctl + b + 1, shadow_count + b, b);
+                         live(b), head(3 * b + 2), w.queue[(b + 1) & 1], live(b + 1), shadow_count(b), b);
       mark(kStageShade, 0);
     }
     // the last bounce's sun queries resolve radiance that the escape tests
@@ -304,13 +333,13 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
     // closest-hit queue is the zero count)
     mark(kStageTrace, 1);
     hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                       shadow_count + (bounces - 1), heads + 3 * bounces, w.queue[bounces & 1],
-                       shadow_count + bounces, heads + 3 * bounces + 1, 0);
+                       shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], shadow_count(bounces),
+                       head(3 * bounces + 1), 0);
     mark(kStageTrace, 0);
     mark(kStageEscape, 1);
     hipLaunchKernelGGL((wf_trace<M, true>), dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                       shadow_count + bounces, heads + 3 * bounces + 2, w.queue[bounces & 1], ctl + bounces,
-                       heads + 3 * bounces + 3, 0);
+                       shadow_count(bounces), head(3 * bounces + 2), w.queue[bounces & 1], live(bounces),
+                       head(3 * bounces + 3), 0);
     mark(kStageEscape, 0);
     mark(kStageFilm, 1);
     hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w,
