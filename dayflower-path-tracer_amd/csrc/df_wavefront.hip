@@ -181,9 +181,9 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
 // (any-hit + sky, :3689-3700).  Both queues drain through the same
 // persistent waves, so a short queue does not leave the GPU idle and a bounce
 // pays one launch tail.  The two queues of one launch never touch the same
-// path data: a shadow query writes the radiance and the sample, a
-// closest-hit query the hit record; an escape test reads the radiance, so
-// the last sun queries run in a launch before it.
+// path data: a shadow query writes the radiance (and the sample of a path
+// that ended), a closest-hit query the hit record, an escape test the sample
+// of a path that took every bounce.
 // Occupancy hint: six waves per SIMD (80 VGPRs, a few spills); the BVH walks
 // are latency-bound, and six measured faster than five or eight.
 template <int M, bool kEscape>
@@ -213,11 +213,19 @@ __global__ __launch_bounds__(kThreads, 6) void wf_trace(DScene S, DCamera C, DSk
     if (!valid) return;
     const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
     if constexpr (kEscape) {
-      const float4 m4 = W.thru[i], p4 = W.rad[i];
-      const float o[3] = {o4.x, o4.y, o4.z}, d[3] = {d4.x, d4.y, d4.z}, m[3] = {m4.x, m4.y, m4.z};
-      float p[3] = {p4.x, p4.y, p4.z};
-      tr.pt_escape(o, d, m, p, sky_active);
-      W.sample[i] = f4(p, 0.0f);
+      // The escape test (:3689-3700) runs beside the last bounce's sun queries,
+      // which may still change this path's radiance: the sample holds the sky
+      // term m * sky / pi (w = 2) or nothing (w = 3), and the film pass adds the
+      // final radiance, the same single float addition the reference makes.
+      const bool hit = tr.template intersect_t<true>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z) < DF_INF;
+      if (!hit) {
+        const float4 m4 = W.thru[i];
+        tr.sky(sky_active, d4.x, d4.y, d4.z);
+        W.sample[i] = make_float4(m4.x * tr.tmp[0] * DF_PI_RECIP, m4.y * tr.tmp[1] * DF_PI_RECIP,
+                                  m4.z * tr.tmp[2] * DF_PI_RECIP, 2.0f);
+      } else {
+        W.sample[i] = make_float4(0.0f, 0.0f, 0.0f, 3.0f);
+      }
     } else {
       int ci, cst, cso;
       const float t = tr.template query<false>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z, ci, cst, cso);
@@ -243,7 +251,12 @@ __global__ __launch_bounds__(256) void wf_film(DScene S, DCamera C, DSky K, DPar
   if (x < P.width && y < P.height) {
     const int gid = y * P.width + x;
     Tracer<M> tr(S, C, K, P, perm);
-    const float4 col = W.sample[gid];
+    float4 col = W.sample[gid];
+    if (col.w >= 2.0f) {  // a path that took every bounce: its radiance, plus the sky term if it escaped
+      const float4 r = W.rad[gid];
+      col = col.w == 2.0f ? make_float4(r.x + col.x, r.y + col.y, r.z + col.z, 0.0f)
+                          : make_float4(r.x, r.y, r.z, 0.0f);
+    }
     const bool set = col.w != 0.0f;
     float film[3] = {F.film[gid * 3 + 0], F.film[gid * 3 + 1], F.film[gid * 3 + 2]};
     int nsamp = F.film_samples[gid];
@@ -328,18 +341,13 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
                          live(b), head(3 * b + 2), w.queue[(b + 1) & 1], live(b + 1), shadow_count(b), b);
       mark(kStageShade, 0);
     }
-    // the last bounce's sun queries resolve radiance that the escape tests
-    // then add to, so they finish first (a launch of their own: its
-    // closest-hit queue is the zero count)
-    mark(kStageTrace, 1);
-    hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                       shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], shadow_count(bounces),
-                       head(3 * bounces + 1), 0);
-    mark(kStageTrace, 0);
+    // the last bounce's sun queries and the escape tests of the paths that
+    // took every bounce, in one launch (they write different path data; the
+    // film pass adds the two)
     mark(kStageEscape, 1);
     hipLaunchKernelGGL((wf_trace<M, true>), dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
-                       shadow_count(bounces), head(3 * bounces + 2), w.queue[bounces & 1], live(bounces),
-                       head(3 * bounces + 3), 0);
+                       shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], live(bounces),
+                       head(3 * bounces + 1), 0);
     mark(kStageEscape, 0);
     mark(kStageFilm, 1);
     hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w,
