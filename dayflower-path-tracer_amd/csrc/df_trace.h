@@ -32,8 +32,8 @@ struct Vtx {
 // kPrefetch: the BVH walk loads both possible successors of a node (its left
 // child and its skip target) before the node's test decides between them.
 // That shortens the chain of dependent loads at the cost of twice the node
-// loads: faster in the low-occupancy megakernel, slower in kernels with the
-// occupancy to hide the latency.
+// loads: faster in the low-occupancy megakernel and adjoint kernels, slower
+// in kernels with the occupancy to hide the latency.
 template <int M, bool kPrefetch = false>
 struct Tracer {
   const DScene& S;
