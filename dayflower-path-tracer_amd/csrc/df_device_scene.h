@@ -28,7 +28,8 @@ struct DPrim {
   float w2o[12];         // world -> object rows 1..3
   float o2w[12];         // object -> world rows 1..3
   float geo[8];          // sphere: pos.xyz, radius; plane: a.xyz, n.xyz; triangle: unused
-  int32_t pad[10];
+  int32_t run_end;       // mesh: one past the last of the consecutive meshes starting here; else index + 1
+  int32_t pad[9];
 };
 
 // World-space bounding sphere of what a primitive's test can hit, for

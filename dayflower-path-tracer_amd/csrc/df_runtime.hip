@@ -478,6 +478,9 @@ int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::v
     }
     cull[i] = cull_bounds(h, nodes, p);
   }
+  for (size_t i = np; i-- > 0;)  // runs of consecutive meshes
+    prims[i].run_end = prims[i].type == 5 && i + 1 < np && prims[i + 1].type == 5 ? prims[i + 1].run_end
+                                                                                  : static_cast<int32_t>(i + 1);
   return DF_OK;
 }
 

@@ -29,13 +29,22 @@ struct Vtx {
   float sun_w, dsun_w;
 };
 
-// kPrefetch: the BVH walk loads both possible successors of a node (its left
-// child and its skip target) before the node's test decides between them.
-// That shortens the chain of dependent loads at the cost of twice the node
-// loads: faster in the low-occupancy megakernel and adjoint kernels, slower
-// in kernels with the occupancy to hide the latency.
-template <int M, bool kPrefetch = false>
+// How the BVH walks are scheduled (results are the same in every mode):
+//   kWalkPlain     each mesh's walk in turn, the wave waiting for its slowest
+//                  lane at the end of each mesh;
+//   kWalkPrefetch  as plain, loading both possible successors of a node (its
+//                  left child and its skip target) before the node's test
+//                  decides between them: a shorter chain of dependent loads
+//                  for twice the node loads, faster in the low-occupancy
+//                  megakernel and adjoint kernels;
+//   kWalkMerged    a run of consecutive meshes walked as one loop of node
+//                  visits per lane (walk_meshes), faster in the wavefront
+//                  trace kernel, slower in the batch kernel.
+constexpr int kWalkPlain = 0, kWalkPrefetch = 1, kWalkMerged = 2;
+
+template <int M, int kWalk = kWalkPlain>
 struct Tracer {
+  static constexpr bool kPrefetch = kWalk == kWalkPrefetch;
   const DScene& S;
   const DCamera& C;
   const DSky& K;
@@ -140,6 +149,70 @@ struct Tracer {
     return closest;
   }
 
+  // The BVH walks (:1097-1194) of the marked meshes `mine` (bits relative to
+  // `base`), in index order, as one loop of node visits.  Updates the closest
+  // hit; returns true when an any-hit query has its hit.
+  __device__ __forceinline__ bool walk_meshes(uint64_t mine, int base, float ox, float oy, float oz, float dx,
+                                              float dy, float dz, float& closest, int& ci, int& cst, int& cso,
+                                              bool testing) {
+    float oxo = 0.0f, oyo = 0.0f, ozo = 0.0f, dxo = 0.0f, dyo = 0.0f, dzo = 0.0f;
+    float rdx = 0.0f, rdy = 0.0f, rdz = 0.0f;
+    int idx = -1, pi = 0;
+    for (;;) {
+      if (idx == -1) {  // this lane's next mesh
+        if (!mine) break;
+        pi = base + __builtin_ctzll(mine);
+        mine &= mine - 1;
+        ++prim_tests;
+        const DPrim& pr = S.prims[pi];
+        const float m0 = pr.w2o[0], m1 = pr.w2o[1], m2 = pr.w2o[2], m3 = pr.w2o[3];
+        const float m4 = pr.w2o[4], m5 = pr.w2o[5], m6 = pr.w2o[6], m7 = pr.w2o[7];
+        const float m8 = pr.w2o[8], m9 = pr.w2o[9], m10 = pr.w2o[10], m11 = pr.w2o[11];
+        oxo = m0 * ox + m1 * oy + m2 * oz + m3;
+        oyo = m4 * ox + m5 * oy + m6 * oz + m7;
+        ozo = m8 * ox + m9 * oy + m10 * oz + m11;
+        dxo = m0 * dx + m1 * dy + m2 * dz;
+        dyo = m4 * dx + m5 * dy + m6 * dz;
+        dzo = m8 * dx + m9 * dy + m10 * dz;
+        rdx = 1.0f / dxo;
+        rdy = 1.0f / dyo;
+        rdz = 1.0f / dzo;
+        idx = pr.node_root;
+      }
+      const DNode nd = at(S.nodes, idx);
+      ++box_tests;
+      const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
+      const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
+      const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
+      const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
+      const float t = tmin > tmax ? -1.0f
+                      : tmin > 0.001f && tmin < closest ? tmin
+                      : tmax > 0.001f && tmax < closest ? tmax
+                                                         : -1.0f;
+      if (t < 0.0f) {
+        idx = nd.next;
+      } else if (nd.tri < 0) {
+        idx = -nd.tri - 2;  // tree node: descend to the left child
+      } else {
+        for (int k = nd.tri;; ++k) {
+          const DTri tr = at(S.tris, k);
+          ++tri_tests;
+          const float cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
+          if (cur < closest) {
+            closest = cur;
+            ci = pi;
+            cst = 4;
+            cso = tr.tri_offset;
+          }
+          if (testing && ci != -1) return true;
+          if (tr.last) break;
+        }
+        idx = nd.next;
+      }
+    }
+    return false;
+  }
+
   // The primitive loop of doIntersectPrimitives: the closest (or, testing,
   // the first) hit's distance, primitive index, shape type and shape offset.
   template <bool testing>
@@ -179,6 +252,21 @@ struct Tracer {
       }
       while (any) {
       const int j = __builtin_ctzll(any);
+      if constexpr (kWalk == kWalkMerged) {
+        // A run of consecutive meshes: each lane walks its own marked meshes of
+        // the run, in index order, in one loop whose steps are single node
+        // visits, so a lane whose walk ends moves straight on to its next mesh
+        // instead of waiting for the wave.  Every lane's own sequence of
+        // tests, and so every result, is the reference's.
+        const DPrim pu = load_uniform(S.prims, base + j);
+        if (pu.type == 5) {
+          const int end = (pu.run_end < base + m ? pu.run_end : base + m) - base;
+          const uint64_t run = (end >= 64 ? ~uint64_t{0} : ((uint64_t{1} << end) - 1)) & ~((uint64_t{1} << j) - 1);
+          any &= ~run;
+          if (walk_meshes(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing)) return closest;
+          continue;
+        }
+      }
       any &= any - 1;
       if (!((cand >> j) & 1)) continue;
       const int i = base + j;
