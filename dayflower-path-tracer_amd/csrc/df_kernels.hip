@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K
   const int x = blockIdx.x * 16 + threadIdx.x;
   const int y = blockIdx.y * 16 + threadIdx.y;
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (x < P.width && y < P.height) render_pixel<M, R, kWalkPrefetch>(S, C, K, P, F, perm, x, y, cnt);
+  if (x < P.width && y < P.height) render_pixel<M, R, kWalkMerged>(S, C, K, P, F, perm, x, y, cnt);
   flush_counters(cnt, F.counters, kStageMega);
 }
 

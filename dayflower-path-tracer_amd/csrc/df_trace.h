@@ -32,19 +32,14 @@ struct Vtx {
 // How the BVH walks are scheduled (results are the same in every mode):
 //   kWalkPlain     each mesh's walk in turn, the wave waiting for its slowest
 //                  lane at the end of each mesh;
-//   kWalkPrefetch  as plain, loading both possible successors of a node (its
-//                  left child and its skip target) before the node's test
-//                  decides between them: a shorter chain of dependent loads
-//                  for twice the node loads, faster in the low-occupancy
-//                  megakernel and adjoint kernels;
 //   kWalkMerged    a run of consecutive meshes walked as one loop of node
-//                  visits per lane (walk_meshes), faster in the wavefront
-//                  trace kernel, slower in the batch kernel.
-constexpr int kWalkPlain = 0, kWalkPrefetch = 1, kWalkMerged = 2;
+//                  visits per lane (walk_meshes): faster in the wavefront
+//                  trace, megakernel and adjoint kernels, slower (measured)
+//                  in the batch kernel.
+constexpr int kWalkPlain = 0, kWalkMerged = 1;
 
 template <int M, int kWalk = kWalkPlain>
 struct Tracer {
-  static constexpr bool kPrefetch = kWalk == kWalkPrefetch;
   const DScene& S;
   const DCamera& C;
   const DSky& K;
@@ -283,83 +278,39 @@ struct Tracer {
       float cur = DF_INF;
       if (st == 5) {
         const float rdx = 1.0f / dxo, rdy = 1.0f / dyo, rdz = 1.0f / dzo;
-        if constexpr (kPrefetch) {
-          int idx = pr.node_root;
-          DNode nd = at(S.nodes, idx);
-          while (idx != -1) {
-            // both possible successors (index 0 stands in for a missing one)
-            const DNode n_left = at(S.nodes, nd.tri < 0 ? -nd.tri - 2 : 0);
-            const DNode n_next = at(S.nodes, nd.next >= 0 ? nd.next : 0);
-            ++box_tests;
-            const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
-            const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
-            const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
-            const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
-            const float t = tmin > tmax ? -1.0f
-                            : tmin > 0.001f && tmin < closest ? tmin
-                            : tmax > 0.001f && tmax < closest ? tmax
-                                                               : -1.0f;
-            if (t < 0.0f) {
-              idx = nd.next;
-              nd = n_next;
-            } else if (nd.tri < 0) {
-              idx = -nd.tri - 2;  // tree node: descend to the left child
-              nd = n_left;
-            } else {
-              int k = nd.tri;
-              for (;;) {
-                const DTri tr = at(S.tris, k);
-                ++tri_tests;
-                cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
-                if (cur < closest) {
-                  closest = cur;
-                  ci = i;
-                  cst = 4;
-                  cso = tr.tri_offset;
-                }
-                if (testing && ci != -1) return closest;
-                if (tr.last) break;
-                ++k;
+        int idx = pr.node_root;
+        while (idx != -1) {
+          const DNode nd = at(S.nodes, idx);
+          ++box_tests;
+          const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
+          const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
+          const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
+          const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
+          const float t = tmin > tmax ? -1.0f
+                          : tmin > 0.001f && tmin < closest ? tmin
+                          : tmax > 0.001f && tmax < closest ? tmax
+                                                             : -1.0f;
+          if (t < 0.0f) {
+            idx = nd.next;
+          } else if (nd.tri < 0) {
+            idx = -nd.tri - 2;  // tree node: descend to the left child
+          } else {
+            int k = nd.tri;
+            for (;;) {
+              const DTri tr = at(S.tris, k);
+              ++tri_tests;
+              cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
+              if (cur < closest) {
+                closest = cur;
+                ci = i;
+                cst = 4;
+                cso = tr.tri_offset;
               }
-              idx = nd.next;
-              nd = n_next;
+              if (testing && ci != -1) return closest;
+              if (tr.last) break;
+              ++k;
             }
-          }
-        } else {
-          int idx = pr.node_root;
-          while (idx != -1) {
-            const DNode nd = at(S.nodes, idx);
-            ++box_tests;
-            const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
-            const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
-            const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
-            const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
-            const float t = tmin > tmax ? -1.0f
-                            : tmin > 0.001f && tmin < closest ? tmin
-                            : tmax > 0.001f && tmax < closest ? tmax
-                                                               : -1.0f;
-            if (t < 0.0f) {
-              idx = nd.next;
-            } else if (nd.tri < 0) {
-              idx = -nd.tri - 2;  // tree node: descend to the left child
-            } else {
-              int k = nd.tri;
-              for (;;) {
-                const DTri tr = at(S.tris, k);
-                ++tri_tests;
-                cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
-                if (cur < closest) {
-                  closest = cur;
-                  ci = i;
-                  cst = 4;
-                  cso = tr.tri_offset;
-                }
-                if (testing && ci != -1) return closest;
-                if (tr.last) break;
-                ++k;
-              }
-              idx = nd.next;
-            }
+            idx = nd.next;
           }
         }
       } else if (st == 1) {
