@@ -8,11 +8,12 @@ intersection query (closest-hit or any-hit), counted on the device.
 
 Besides the contract fields the JSON line carries:
   roofline     - the dominant kernel's (the stage with the most kernel time)
-                 algorithmic FLOP rate: its per-launch work counts x per-unit
-                 FLOP figures over its average launch duration, both measured
-                 live (device counters; HIP events around every launch on the
-                 renderer's stream), against the FP32 peak; the byte figure is
-                 reported alongside;
+                 algorithmic rate: its per-launch work counts x per-unit byte
+                 and FLOP figures over its average launch duration, both
+                 measured live (device counters; HIP events around every
+                 launch on the renderer's stream), against the HBM peak when
+                 its FLOP per byte is below the FP32 / HBM balance point,
+                 else against the FP32 peak;
   stages       - kernel time, launches and ray queries per step of each stage;
   batch        - the batch mix (256 scenes of 64^2..512^2 pixels and 8..128 samples per
                  pixel): Mrays/s rendering the scenes one after another (plain)
@@ -313,16 +314,27 @@ def main():
     alg_bytes = sum(BYTES_PER[n] * sc_[n] for n in BYTES_PER) / st["launches"]
     achieved_tflops = flop / avg_launch_s / 1e12
     achieved_gbs = alg_bytes / avg_launch_s / 1e9
+    # the roofline that bounds the kernel: HBM below the FP32 / HBM balance point
+    intensity = flop / alg_bytes if alg_bytes else float("inf")
+    balance = PEAK_FP32_TFLOPS * 1e12 / (PEAK_HBM_GBS * 1e9)
+    if intensity < balance:
+        bound, achieved, peak, unit = "hbm", achieved_gbs, PEAK_HBM_GBS, "GB/s"
+        note = ("%.3g FLOP per algorithmic byte, below the FP32 / HBM balance of %.3g: the HBM roofline bounds it "
+                "(FP32 side: %.4g TFLOP/s = %.4f of %.4g; no matrix work)"
+                % (intensity, balance, achieved_tflops, achieved_tflops / PEAK_FP32_TFLOPS, PEAK_FP32_TFLOPS))
+    else:
+        bound, achieved, peak, unit = "mfma", achieved_tflops, PEAK_FP32_TFLOPS, "TFLOP/s"
+        note = ("%.3g FLOP per algorithmic byte, above the FP32 / HBM balance of %.3g: the FP32 peak bounds it "
+                "(no matrix work)" % (intensity, balance))
     roofline = {
-        "bound": "mfma",
-        "achieved": round(achieved_tflops, 4),
-        "peak": PEAK_FP32_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 6),
+        "bound": bound,
+        "achieved": round(achieved, 4),
+        "peak": peak,
+        "unit": unit,
+        "frac": round(achieved / peak, 6),
         "traffic": traffic_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, "")),
         "kernel": KERNEL_NAMES[dom],
-        "note": "VALU-bound (no matrix work): peak is the FP32 figure; algorithmic bytes %.4g GB/s = %.4f of HBM"
-                % (achieved_gbs, achieved_gbs / PEAK_HBM_GBS),
+        "note": note,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),
         "launches_per_step": st["launches"] / steps,
         "flop_per_launch": flop,
