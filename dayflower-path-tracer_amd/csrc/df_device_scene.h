@@ -17,9 +17,7 @@
 
 namespace dfk {
 
-constexpr int kPrimFloats = 36;  // DPrim payload size in 32-bit words
-
-// One primitive, 36 words (144 bytes).
+// One primitive, 46 words (184 bytes).
 struct DPrim {
   int32_t type;          // shape type (1 plane, 2 sphere, 3 terrain, 4 triangle, 5 mesh)
   int32_t shape_offset;  // reference shape offset (planes / spheres / triangles / BVH segment)
@@ -29,8 +27,10 @@ struct DPrim {
   float o2w[12];         // object -> world rows 1..3
   float geo[8];          // sphere: pos.xyz, radius; plane: a.xyz, n.xyz; triangle: unused
   int32_t run_end;       // mesh: one past the last of the consecutive meshes starting here; else index + 1
-  int32_t pad[9];
+  int32_t xform;         // index of the first primitive with the same w2o (bit for bit)
+  int32_t pad[8];
 };
+static_assert(sizeof(DPrim) == 184, "DPrim layout");
 
 // World-space bounding sphere of what a primitive's test can hit, for
 // skipping primitives a ray cannot reach.  on = 0: never skipped (planes,

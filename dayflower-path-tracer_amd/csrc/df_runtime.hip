@@ -478,6 +478,15 @@ int build_prims(df_ctx* c, const HostScene& h, std::vector<DPrim>& prims, std::v
     }
     cull[i] = cull_bounds(h, nodes, p);
   }
+  // transform classes: primitives whose world -> object rows are equal bit
+  // for bit map a ray to the same object-space ray
+  {
+    std::unordered_map<std::string, int32_t> first_of;
+    for (size_t i = 0; i < np; ++i) {
+      const std::string key(reinterpret_cast<const char*>(prims[i].w2o), sizeof prims[i].w2o);
+      prims[i].xform = first_of.emplace(key, static_cast<int32_t>(i)).first->second;
+    }
+  }
   for (size_t i = np; i-- > 0;)  // runs of consecutive meshes
     prims[i].run_end = prims[i].type == 5 && i + 1 < np && prims[i + 1].type == 5 ? prims[i + 1].run_end
                                                                                   : static_cast<int32_t>(i + 1);

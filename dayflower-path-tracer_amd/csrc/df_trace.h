@@ -152,7 +152,7 @@ struct Tracer {
                                               bool testing) {
     float oxo = 0.0f, oyo = 0.0f, ozo = 0.0f, dxo = 0.0f, dyo = 0.0f, dzo = 0.0f;
     float rdx = 0.0f, rdy = 0.0f, rdz = 0.0f;
-    int idx = -1, pi = 0;
+    int idx = -1, pi = 0, xform = -1;
     for (;;) {
       if (idx == -1) {  // this lane's next mesh
         if (!mine) break;
@@ -160,19 +160,24 @@ struct Tracer {
         mine &= mine - 1;
         ++prim_tests;
         const DPrim& pr = S.prims[pi];
-        const float m0 = pr.w2o[0], m1 = pr.w2o[1], m2 = pr.w2o[2], m3 = pr.w2o[3];
-        const float m4 = pr.w2o[4], m5 = pr.w2o[5], m6 = pr.w2o[6], m7 = pr.w2o[7];
-        const float m8 = pr.w2o[8], m9 = pr.w2o[9], m10 = pr.w2o[10], m11 = pr.w2o[11];
-        oxo = m0 * ox + m1 * oy + m2 * oz + m3;
-        oyo = m4 * ox + m5 * oy + m6 * oz + m7;
-        ozo = m8 * ox + m9 * oy + m10 * oz + m11;
-        dxo = m0 * dx + m1 * dy + m2 * dz;
-        dyo = m4 * dx + m5 * dy + m6 * dz;
-        dzo = m8 * dx + m9 * dy + m10 * dz;
-        rdx = 1.0f / dxo;
-        rdy = 1.0f / dyo;
-        rdz = 1.0f / dzo;
         idx = pr.node_root;
+        // The object-space ray and its reciprocals, unless the previous mesh
+        // had the same transform (then they are these bits already).
+        if (pr.xform != xform) {
+          xform = pr.xform;
+          const float m0 = pr.w2o[0], m1 = pr.w2o[1], m2 = pr.w2o[2], m3 = pr.w2o[3];
+          const float m4 = pr.w2o[4], m5 = pr.w2o[5], m6 = pr.w2o[6], m7 = pr.w2o[7];
+          const float m8 = pr.w2o[8], m9 = pr.w2o[9], m10 = pr.w2o[10], m11 = pr.w2o[11];
+          oxo = m0 * ox + m1 * oy + m2 * oz + m3;
+          oyo = m4 * ox + m5 * oy + m6 * oz + m7;
+          ozo = m8 * ox + m9 * oy + m10 * oz + m11;
+          dxo = m0 * dx + m1 * dy + m2 * dz;
+          dyo = m4 * dx + m5 * dy + m6 * dz;
+          dzo = m8 * dx + m9 * dy + m10 * dz;
+          rdx = 1.0f / dxo;
+          rdy = 1.0f / dyo;
+          rdz = 1.0f / dzo;
+        }
       }
       const DNode nd = at(S.nodes, idx);
       ++box_tests;
