@@ -127,7 +127,7 @@ __device__ __forceinline__ void through_beta(double* g, const PathFactors& pf, c
 // The sky term beta * sky(d) / pi of a path that escaped: through the albedos
 // in beta, and through turbidity and elevation (central differences).
 template <int M>
-__device__ __forceinline__ void sky_term(Tracer<M, kWalkMerged>& tr, double* g, const DAdjoint& A, const PathFactors& pf,
+__device__ __forceinline__ void sky_term(Tracer<M>& tr, double* g, const DAdjoint& A, const PathFactors& pf,
                                          const float w[3], const float beta[3], const float d[3], bool sky_active) {
   tr.sky(sky_active, d[0], d[1], d[2]);
   const float tail[3] = {tr.tmp[0] * DF_PI_RECIP, tr.tmp[1] * DF_PI_RECIP, tr.tmp[2] * DF_PI_RECIP};
@@ -151,7 +151,7 @@ __device__ __forceinline__ void sky_term(Tracer<M, kWalkMerged>& tr, double* g, 
 
 // One path of render_pt (:3151-3705) replayed with the adjoint bookkeeping.
 template <int M>
-__device__ void replay_path(Tracer<M, kWalkMerged>& tr, double* g, const DAdjoint& A, const float ray[6], const float w[3],
+__device__ void replay_path(Tracer<M>& tr, double* g, const DAdjoint& A, const float ray[6], const float w[3],
                             bool sky_active, bool sun_active) {
   const int depth_max = tr.P.depth_max;
   int depth = 0;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(256, 3) void adjoint_kernel(DScene S, DCamera C, DS
   const int y = blockIdx.y * 16 + threadIdx.y;
   if (x < P.width && y < P.height) {
     const int gid = y * P.width + x;
-    Tracer<M, kWalkMerged> tr(S, C, K, P, perm);
+    Tracer<M> tr(S, C, K, P, perm);
     tr.rng.s = F.seeds[gid];
     const float w[3] = {A.weight[gid * 3 + 0], A.weight[gid * 3 + 1], A.weight[gid * 3 + 2]};
     const bool sky_active = K.sky_active == 1;

@@ -18,12 +18,12 @@ namespace dfk {
 // the tone-mapped pixel.  R is the renderer type (1 AO, 2 path tracer, 3 ray
 // caster, 4 ray marcher, 5 ray tracer, 6 surface normals), a template
 // parameter so that each instantiation holds only its own ray queries.
-template <int M, int R, int kWalk>
+template <int M, int R>
 __device__ __forceinline__ void render_pixel(const DScene& S, const DCamera& C, const DSky& K, const DParams& P,
                                              const DFrame& F, const int* perm, int x, int y,
                                              unsigned long long cnt[kCounterCount]) {
   const int gid = y * P.width + x;
-  Tracer<M, kWalk> tr(S, C, K, P, perm);
+  Tracer<M> tr(S, C, K, P, perm);
   tr.rng.s = F.seeds[gid];
   float film[3] = {F.film[gid * 3 + 0], F.film[gid * 3 + 1], F.film[gid * 3 + 2]};
   int nsamp = F.film_samples[gid];
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void render_kernel(DScene S, DCamera C, DSky K
   const int x = blockIdx.x * 16 + threadIdx.x;
   const int y = blockIdx.y * 16 + threadIdx.y;
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
-  if (x < P.width && y < P.height) render_pixel<M, R, kWalkMerged>(S, C, K, P, F, perm, x, y, cnt);
+  if (x < P.width && y < P.height) render_pixel<M, R>(S, C, K, P, F, perm, x, y, cnt);
   flush_counters(cnt, F.counters, kStageMega);
 }
 
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256, 6) void batch_kernel(const DJob* __restrict__ 
     const int x = item.tile_x * 16 + threadIdx.x;
     const int y = item.tile_y * 16 + threadIdx.y;
     unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (x < J.P.width && y < J.P.height) render_pixel<M, R, kWalkPlain>(J.S, J.C, J.K, J.P, J.F, perm, x, y, cnt);
+    if (x < J.P.width && y < J.P.height) render_pixel<M, R>(J.S, J.C, J.K, J.P, J.F, perm, x, y, cnt);
     flush_counters(cnt, J.F.counters, kStageMega);
   }
 }

@@ -95,8 +95,8 @@ __device__ __forceinline__ void flush_counters(const unsigned long long cnt[kCou
   }
 }
 
-template <int M, int W>
-__device__ __forceinline__ void take_counters(const Tracer<M, W>& tr, unsigned long long cnt[kCounterCount]) {
+template <int M>
+__device__ __forceinline__ void take_counters(const Tracer<M>& tr, unsigned long long cnt[kCounterCount]) {
   cnt[0] += tr.rays;
   cnt[1] += tr.samples;
   cnt[2] += tr.box_tests;

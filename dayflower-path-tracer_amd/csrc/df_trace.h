@@ -29,21 +29,12 @@ struct Vtx {
   float sun_w, dsun_w;
 };
 
-// How the BVH walks are scheduled (results are the same in every mode):
-//   kWalkPlain     each mesh's walk in turn, the wave waiting for its slowest
-//                  lane at the end of each mesh;
-//   kWalkMerged    a run of consecutive meshes walked as one loop of node
-//                  visits per lane (walk_meshes): faster in the wavefront
-//                  trace, megakernel and adjoint kernels, slower (measured)
-//                  in the batch kernel.
-constexpr int kWalkPlain = 0, kWalkMerged = 1;
-
-// The merged walk tests leaves' triangles once this fraction of the lanes
+// The mesh walk tests leaves' triangles once this fraction of the lanes
 // still walking wait at a leaf (measured best on the 1080p workload: 1/8,
 // 1/6, 1/2, 3/4 and all slower).
 constexpr int kLeafWaitNum = 1, kLeafWaitDen = 4;
 
-template <int M, int kWalk = kWalkPlain>
+template <int M>
 struct Tracer {
   const DScene& S;
   const DCamera& C;
@@ -271,93 +262,54 @@ struct Tracer {
         }
       }
       while (any) {
-      const int j = __builtin_ctzll(any);
-      if constexpr (kWalk == kWalkMerged) {
-        // A run of consecutive meshes: each lane walks its own marked meshes of
-        // the run, in index order, in one loop whose steps are single node
-        // visits, so a lane whose walk ends moves straight on to its next mesh
-        // instead of waiting for the wave.  Every lane's own sequence of
-        // tests, and so every result, is the reference's.
-        const DPrim pu = load_uniform(S.prims, base + j);
-        if (pu.type == 5) {
-          const int end = (pu.run_end < base + m ? pu.run_end : base + m) - base;
+        const int j = __builtin_ctzll(any);
+        const DPrim pr = load_uniform(S.prims, base + j);
+        if (pr.type == 5) {
+          // A run of consecutive meshes (a single mesh is a run of one): each
+          // lane walks its own marked meshes of the run, in index order, in one
+          // loop whose steps are single node visits, so a lane whose walk ends
+          // moves straight on to its next mesh instead of waiting for the wave.
+          // Every lane's own sequence of tests, and so every result, is the
+          // reference's.
+          const int end = (pr.run_end < base + m ? pr.run_end : base + m) - base;
           const uint64_t run = (end >= 64 ? ~uint64_t{0} : ((uint64_t{1} << end) - 1)) & ~((uint64_t{1} << j) - 1);
           any &= ~run;
           if (walk_meshes(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing)) return closest;
           continue;
         }
-      }
-      any &= any - 1;
-      if (!((cand >> j) & 1)) continue;
-      const int i = base + j;
-      const DPrim pr = load_uniform(S.prims, i);
-      const int st = pr.type;
-      ++prim_tests;
-      const float* m = pr.w2o;
-      const float oxo = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
-      const float oyo = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
-      const float ozo = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
-      const float dxo = m[0] * dx + m[1] * dy + m[2] * dz;
-      const float dyo = m[4] * dx + m[5] * dy + m[6] * dz;
-      const float dzo = m[8] * dx + m[9] * dy + m[10] * dz;
-      float cur = DF_INF;
-      if (st == 5) {
-        const float rdx = 1.0f / dxo, rdy = 1.0f / dyo, rdz = 1.0f / dzo;
-        int idx = pr.node_root;
-        while (idx != -1) {
-          const DNode nd = at(S.nodes, idx);
-          ++box_tests;
-          const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
-          const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
-          const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
-          const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
-          const float t = tmin > tmax ? -1.0f
-                          : tmin > 0.001f && tmin < closest ? tmin
-                          : tmax > 0.001f && tmax < closest ? tmax
-                                                             : -1.0f;
-          if (t < 0.0f) {
-            idx = nd.next;
-          } else if (nd.tri < 0) {
-            idx = -nd.tri - 2;  // tree node: descend to the left child
-          } else {
-            int k = nd.tri;
-            for (;;) {
-              const DTri tr = at(S.tris, k);
-              ++tri_tests;
-              cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
-              if (cur < closest) {
-                closest = cur;
-                ci = i;
-                cst = 4;
-                cso = tr.tri_offset;
-              }
-              if (testing && ci != -1) return closest;
-              if (tr.last) break;
-              ++k;
-            }
-            idx = nd.next;
-          }
+        any &= any - 1;
+        if (!((cand >> j) & 1)) continue;
+        const int i = base + j;
+        const int st = pr.type;
+        ++prim_tests;
+        const float* m = pr.w2o;
+        const float oxo = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
+        const float oyo = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
+        const float ozo = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
+        const float dxo = m[0] * dx + m[1] * dy + m[2] * dz;
+        const float dyo = m[4] * dx + m[5] * dy + m[6] * dz;
+        const float dzo = m[8] * dx + m[9] * dy + m[10] * dz;
+        float cur = DF_INF;
+        if (st == 1) {
+          cur = isect_plane(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], pr.geo[4],
+                            pr.geo[5], closest);
+        } else if (st == 2) {
+          ++sphere_tests;
+          cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
+        } else if (st == 3) {
+          cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n, &terrain_steps);
+        } else if (st == 4) {
+          const DTri tr = S.tris[pr.node_root];
+          ++tri_tests;
+          cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
         }
-      } else if (st == 1) {
-        cur = isect_plane(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], pr.geo[4],
-                          pr.geo[5], closest);
-      } else if (st == 2) {
-        ++sphere_tests;
-        cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
-      } else if (st == 3) {
-        cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n, &terrain_steps);
-      } else if (st == 4) {
-        const DTri tr = S.tris[pr.node_root];
-        ++tri_tests;
-        cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
-      }
-      if (cur < closest) {
-        closest = cur;
-        ci = i;
-        cst = st;
-        cso = pr.shape_offset;
-      }
-      if (testing && ci != -1) return closest;
+        if (cur < closest) {
+          closest = cur;
+          ci = i;
+          cst = st;
+          cso = pr.shape_offset;
+        }
+        if (testing && ci != -1) return closest;
       }
     }
     return closest;

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kThreads, 6) void wf_trace(DScene S, DCamera C, DSk
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce) {
-  Tracer<M, kWalkMerged> tr(S, C, K, P, kPerm);
+  Tracer<M> tr(S, C, K, P, kPerm);
   drain(W.shadow_queue, W.n, shadow_count, shadow_head, [&](bool valid, int32_t e) {
     if (!valid) return;
     const int32_t i = e & (kFinish - 1);
