@@ -208,18 +208,15 @@ struct Tracer {
       const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
       const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
       const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
-      const float t = tmin > tmax ? -1.0f
-                      : tmin > 0.001f && tmin < closest ? tmin
-                      : tmax > 0.001f && tmax < closest ? tmax
-                                                         : -1.0f;
-      if (t < 0.0f) {
-        idx = nd.next;
-      } else if (nd.tri < 0) {
-        idx = -nd.tri - 2;  // tree node: descend to the left child
-      } else {
-        leaf = nd.tri;  // its triangles come before the next visit
-        idx = nd.next;
-      }
+      // The box is entered when (:1122-1134) its entry or exit lies in
+      // (0.001, closest), unless tmin > tmax (false for NaN); written without
+      // branches, as selects.
+      const int in_min = static_cast<int>(tmin > 0.001f) & static_cast<int>(tmin < closest);
+      const int in_max = static_cast<int>(tmax > 0.001f) & static_cast<int>(tmax < closest);
+      const bool hit = static_cast<bool>(static_cast<int>(!(tmin > tmax)) & (in_min | in_max));
+      const bool tree = nd.tri < 0;
+      leaf = hit && !tree ? nd.tri : -1;                   // a leaf's triangles come before the next visit
+      idx = hit && tree ? -nd.tri - 2 : nd.next;           // tree node: descend to the left child
     }
     return false;
   }
