@@ -184,10 +184,11 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
 // path data: a shadow query writes the radiance (and the sample of a path
 // that ended), a closest-hit query the hit record, an escape test the sample
 // of a path that took every bounce.
-// Occupancy hint: six waves per SIMD (80 VGPRs, a few spills); the BVH walks
-// are latency-bound, and six measured faster than five or eight.
+// Occupancy hint: seven waves per SIMD for the bounces' queries, six for the
+// last launch's (a few registers spilled); measured faster than five or
+// eight, and than one hint for both.
 template <int M, bool kEscape>
-__global__ __launch_bounds__(kThreads, 6) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+__global__ __launch_bounds__(kThreads, kEscape ? 6 : 7) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce) {
