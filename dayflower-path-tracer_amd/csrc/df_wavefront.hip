@@ -14,17 +14,19 @@
 //                                 roulette, scattered ray; appends the path
 //                                 to the next bounce's queue and its sun
 //                                 sample to the shadow queue
-//   trace            the last bounce's sun samples
-//   escape           the sky test of the paths that reached the maximum depth
-//   film             film accumulation, wireframes and the image pass
+//   trace            the last bounce's sun samples, then the escape test
+//                    (any-hit + sky) of the paths that took every bounce
+//   film             film accumulation (adding an escaped path's sky term),
+//                    wireframes and the image pass
 //
 // Every path draws from its pixel's LCG stream in the reference's order
 // (raygen, then each shade), and the sun term is resolved from the two
 // radiance values computed by shade, so a pass gives the megakernel's bits.
-// The trace kernels are persistent: each wave takes 64 queue entries at a
-// time from an atomic head until the queue is drained, which balances the
-// very uneven cost of incoherent rays.  Queues are appended with one atomic
-// per wave.  Path state is indexed by pixel and lives in DPaths.
+// The trace and shade kernels are persistent: each wave takes 64 queue
+// entries at a time from a work head until the queue is drained, which
+// balances the very uneven cost of incoherent rays.  Queues are striped over
+// kQueueStripes lines and appended with one atomic per wave.  Path state is
+// indexed by pixel and lives in DPaths.
 #include <hip/hip_runtime.h>
 
 #include "df_kernels.h"
