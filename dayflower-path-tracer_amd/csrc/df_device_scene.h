@@ -217,6 +217,8 @@ struct DPaths {
   float4* ray0_d;
   int32_t* queue[2];  // live paths of even / odd bounces (kQueueStripes stripes of n entries)
   int32_t* shadow_queue;
+  int32_t* noise_queue;  // a bounce's paths whose hit surface has noise normal mapping
+  float4* noise;         // their three normal-map fBm values (x, y, z)
   int32_t* ctl;       // per-pass queue counts and work heads (groups of kCtlGroup ints)
   int64_t n;          // slots; also the capacity of one queue stripe
   int32_t ctl_len;
@@ -232,11 +234,12 @@ constexpr int kLineInts = 16;                          // one 64-byte line
 constexpr int kCtlGroup = kQueueStripes * kLineInts;  // one count or head per stripe
 
 // Length of DPaths::ctl for a maximum depth, in ints: B + 1 live-path counts,
-// B + 1 shadow counts (the last stays zero) and 3B + 4 work heads, B =
-// max(1, depth), each a group of per-stripe lines.
+// B + 1 shadow counts (the last stays zero), 3B + 4 work heads and per
+// bounce the noise queue's count and work head, B = max(1, depth), each a
+// group of per-stripe lines.
 __host__ __device__ inline int wavefront_ctl_len(int depth_max) {
   const int b = depth_max > 1 ? depth_max : 1;
-  return (5 * b + 6) * kCtlGroup;
+  return (7 * b + 6) * kCtlGroup;
 }
 
 }  // namespace dfk

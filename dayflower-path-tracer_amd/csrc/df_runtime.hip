@@ -69,7 +69,7 @@ struct df_ctx {
   bool clear_pending = false;
 
   // wavefront path state (allocated on first use for the frame size)
-  DevBuf path_f4[11], path_q[3], path_ctl;
+  DevBuf path_f4[12], path_q[4], path_ctl;
   dfk::DPaths paths{};
 
   // path-tracer schedule chosen by the autotuner for DF_PT_AUTO (0: not yet)
@@ -687,7 +687,8 @@ void free_paths(df_ctx* c) {
 }
 
 // Path state of the wavefront path tracer for the current frame size and a
-// maximum depth (11 float4 records + 3 queue entries per pixel: 188 bytes).
+// maximum depth: per pixel 12 float4 records and an entry in every stripe of
+// 4 queues (448 bytes).
 int ensure_paths(df_ctx* c, int depth_max) {
   const size_t n = pixels(c);
   const int ctl_len = dfk::wavefront_ctl_len(depth_max);
@@ -702,12 +703,13 @@ int ensure_paths(df_ctx* c, int depth_max) {
   const int ctl_alloc = ctl_len > 64 * dfk::kCtlGroup ? ctl_len : 64 * dfk::kCtlGroup;
   if ((rc = alloc_buf(c, c->path_ctl, static_cast<size_t>(ctl_alloc) * sizeof(int32_t)))) return rc;
   dfk::DPaths& w = c->paths;
-  float4** f4[11] = {&w.ray_o, &w.ray_d, &w.thru, &w.rad, &w.hit, &w.sh_o, &w.sh_d, &w.rad_lit, &w.sample,
-                     &w.ray0_o, &w.ray0_d};
-  for (int i = 0; i < 11; ++i) *f4[i] = static_cast<float4*>(c->path_f4[i].p);
+  float4** f4[12] = {&w.ray_o, &w.ray_d, &w.thru, &w.rad, &w.hit, &w.sh_o, &w.sh_d, &w.rad_lit, &w.sample,
+                     &w.ray0_o, &w.ray0_d, &w.noise};
+  for (int i = 0; i < 12; ++i) *f4[i] = static_cast<float4*>(c->path_f4[i].p);
   w.queue[0] = static_cast<int32_t*>(c->path_q[0].p);
   w.queue[1] = static_cast<int32_t*>(c->path_q[1].p);
   w.shadow_queue = static_cast<int32_t*>(c->path_q[2].p);
+  w.noise_queue = static_cast<int32_t*>(c->path_q[3].p);
   w.ctl = static_cast<int32_t*>(c->path_ctl.p);
   w.n = static_cast<int64_t>(n);
   w.ctl_len = ctl_alloc;

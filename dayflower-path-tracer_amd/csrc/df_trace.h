@@ -41,6 +41,12 @@ struct Tracer {
   const DSky& K;
   const DParams& P;
   Noise noise;
+  // Noise normal map values precomputed per path (wavefront shading): when
+  // noise_in is set, normal_map_noise reads noise_in[noise_index]; with
+  // noise_skip it leaves the normal unperturbed (the pass computing them).
+  const float4* noise_in = nullptr;
+  int noise_index = 0;
+  bool noise_skip = false;
   Rng rng;
   Hit h;
   float tmp[3];
@@ -494,16 +500,40 @@ struct Tracer {
     h.tan[2] = w[2] * tx + w[6] * ty + w[10] * tz;
   }
 
+  // Whether the surface of primitive ci perturbs the shading normal with
+  // noise (the test of normal_map_noise).
+  __device__ __forceinline__ bool has_noise_normals(int ci) const {
+    if (P.normal_mapping != 1 || ci < 0) return false;
+    const int so = S.prims[ci].surface_offset;
+    return S.surfaces[so + 4] > 0.0f && S.surfaces[so + 5] > 0.0f;
+  }
+
+  // The three fBm values of the noise normal map at the hit point, for a
+  // surface of noise scale `scale`.
+  __device__ __forceinline__ void noise_normal_values(float scale, float n[3]) {
+    const float sr = 1.0f / scale;
+    const float x1 = h.p[0] * sr, y1 = h.p[1] * sr, z1 = h.p[2] * sr;
+    n[0] = noise.fbm3(scale, 0.5f, -0.26f, 0.26f, 16, x1, y1, z1);
+    n[1] = noise.fbm3(scale, 0.5f, -0.26f, 0.26f, 16, y1, z1, x1);
+    n[2] = noise.fbm3(scale, 0.5f, -0.26f, 0.26f, 16, z1, x1, y1);
+  }
+
   __device__ __forceinline__ void normal_map_noise(const DPrim& pr) {  // :2658-2712
     const int so = pr.surface_offset;
     const float amount = S.surfaces[so + 4];
     const float scale = S.surfaces[so + 5];
     if (P.normal_mapping == 1 && amount > 0.0f && scale > 0.0f) {
-      const float sr = 1.0f / scale;
-      const float x1 = h.p[0] * sr, y1 = h.p[1] * sr, z1 = h.p[2] * sr;
-      const float nx = noise.fbm3(scale, 0.5f, -0.26f, 0.26f, 16, x1, y1, z1);
-      const float ny = noise.fbm3(scale, 0.5f, -0.26f, 0.26f, 16, y1, z1, x1);
-      const float nz = noise.fbm3(scale, 0.5f, -0.26f, 0.26f, 16, z1, x1, y1);
+      if (noise_skip) return;  // the caller computes the values itself
+      float n[3];
+      if (noise_in) {  // computed by an earlier pass, from the same hit point
+        const float4 q = noise_in[noise_index];
+        n[0] = q.x;
+        n[1] = q.y;
+        n[2] = q.z;
+      } else {
+        noise_normal_values(scale, n);
+      }
+      const float nx = n[0], ny = n[1], nz = n[2];
       const float s1X = h.ns[0] + nx * amount;
       const float s1Y = h.ns[1] + ny * amount;
       const float s1Z = h.ns[2] + nz * amount;

@@ -139,11 +139,13 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
   __shared__ int perm[512];
   stage_perm(perm);
   Tracer<M> tr(S, C, K, P, perm);
+  tr.noise_in = W.noise;  // computed by wf_normals
   const bool sky_active = K.sky_active == 1;
   const bool sun_active = K.sun_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     bool more = false, shadow = false;
     if (valid) {
+      tr.noise_index = i;
       const float4 o4 = W.ray_o[i], d4 = W.ray_d[i], hv = W.hit[i], m4 = W.thru[i], p4 = W.rad[i];
       float o[3] = {o4.x, o4.y, o4.z}, d[3] = {d4.x, d4.y, d4.z};
       float m[3] = {m4.x, m4.y, m4.z}, p[3] = {p4.x, p4.y, p4.z};
@@ -176,6 +178,31 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
   flush_counters(cnt, F.counters, kStageShade);
 }
 
+// Noise normal maps (:2658-2712) of the paths the trace kernel queued,
+// computed apart from shading: three 16-octave fBm evaluations at the hit
+// point, the most expensive step of shading a hit, needed by the paths that
+// hit such a surface.  Inside the shade kernel a wave ran them for those few
+// lanes while the others waited (1.2 of its 3.2 ms per 1080p pass); here
+// every lane has one.  The shade kernel takes the values from W.noise at
+// the same step of its arithmetic, so its results are unchanged.
+template <int M>
+__global__ __launch_bounds__(kThreads) void wf_normals(DScene S, DCamera C, DSky K, DParams P, DPaths W,
+                                                       const int32_t* count, int32_t* head) {
+  __shared__ int perm[512];
+  stage_perm(perm);
+  Tracer<M> tr(S, C, K, P, perm);
+  tr.noise_skip = true;  // fill_hit's own noise step: computed below instead
+  drain(W.noise_queue, W.n, count, head, [&](bool valid, int32_t i) {
+    if (!valid) return;
+    const float4 o = W.ray_o[i], d = W.ray_d[i], hv = W.hit[i];
+    const int ci = __float_as_int(hv.y);
+    tr.fill_hit(o.x, o.y, o.z, d.x, d.y, d.z, hv.x, ci, __float_as_int(hv.z), __float_as_int(hv.w));
+    float n[3];
+    tr.noise_normal_values(S.surfaces[S.prims[ci].surface_offset + 5], n);
+    W.noise[i] = make_float4(n[0], n[1], n[2], 0.0f);
+  });
+}
+
 // The ray queries of one bounce in one launch: first the sun's shadow
 // queries of the previous bounce (any-hit; each resolves that bounce's
 // radiance, and finishes the path if it ended there), then the closest-hit
@@ -193,7 +220,7 @@ template <int M, bool kEscape>
 __global__ __launch_bounds__(kThreads, kEscape ? 6 : 7) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
-                                                     int first_bounce) {
+                                                     int first_bounce, int32_t* noise_count) {
   Tracer<M> tr(S, C, K, P, kPerm);
   drain(W.shadow_queue, W.n, shadow_count, shadow_head, [&](bool valid, int32_t e) {
     if (!valid) return;
@@ -213,28 +240,32 @@ __global__ __launch_bounds__(kThreads, kEscape ? 6 : 7) void wf_trace(DScene S, 
   });
   const bool sky_active = K.sky_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
-    if (!valid) return;
-    const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
-    if constexpr (kEscape) {
-      // The escape test (:3689-3700) runs beside the last bounce's sun queries,
-      // which may still change this path's radiance: the sample holds the sky
-      // term m * sky / pi (w = 2) or nothing (w = 3), and the film pass adds the
-      // final radiance, the same single float addition the reference makes.
-      const bool hit = tr.template intersect_t<true>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z) < DF_INF;
-      if (!hit) {
-        const float4 m4 = W.thru[i];
-        tr.sky(sky_active, d4.x, d4.y, d4.z);
-        W.sample[i] = make_float4(m4.x * tr.tmp[0] * DF_PI_RECIP, m4.y * tr.tmp[1] * DF_PI_RECIP,
-                                  m4.z * tr.tmp[2] * DF_PI_RECIP, 2.0f);
+    bool noisy = false;
+    if (valid) {
+      const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
+      if constexpr (kEscape) {
+        // The escape test (:3689-3700) runs beside the last bounce's sun queries,
+        // which may still change this path's radiance: the sample holds the sky
+        // term m * sky / pi (w = 2) or nothing (w = 3), and the film pass adds the
+        // final radiance, the same single float addition the reference makes.
+        const bool hit = tr.template intersect_t<true>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z) < DF_INF;
+        if (!hit) {
+          const float4 m4 = W.thru[i];
+          tr.sky(sky_active, d4.x, d4.y, d4.z);
+          W.sample[i] = make_float4(m4.x * tr.tmp[0] * DF_PI_RECIP, m4.y * tr.tmp[1] * DF_PI_RECIP,
+                                    m4.z * tr.tmp[2] * DF_PI_RECIP, 2.0f);
+        } else {
+          W.sample[i] = make_float4(0.0f, 0.0f, 0.0f, 3.0f);
+        }
       } else {
-        W.sample[i] = make_float4(0.0f, 0.0f, 0.0f, 3.0f);
+        int ci, cst, cso;
+        const float t = tr.template query<false>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z, ci, cst, cso);
+        W.hit[i] = make_float4(t, __int_as_float(ci), __int_as_float(cst), __int_as_float(cso));
+        if (first_bounce) F.primitive_offsets[i] = ci == -1 ? -1 : ci * 3;
+        noisy = tr.has_noise_normals(ci);
       }
-    } else {
-      int ci, cst, cso;
-      const float t = tr.template query<false>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z, ci, cst, cso);
-      W.hit[i] = make_float4(t, __int_as_float(ci), __int_as_float(cst), __int_as_float(cso));
-      if (first_bounce) F.primitive_offsets[i] = ci == -1 ? -1 : ci * 3;
     }
+    if constexpr (!kEscape) wave_append(W.noise_queue, W.n, noise_count, noisy, i);
   });
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   take_counters(tr, cnt);
@@ -311,6 +342,7 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
   static const int g_trace = resident_blocks(wf_trace<M, false>);
   static const int g_shade = resident_blocks(wf_shade<M>);
   static const int g_escape = resident_blocks(wf_trace<M, true>);
+  static const int g_normals = resident_blocks(wf_normals<M>);
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
@@ -322,6 +354,8 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
   auto live = [&](int b) { return group(b); };
   auto shadow_count = [&](int b) { return group(bounces + 1 + b); };
   auto head = [&](int h) { return group(2 * bounces + 2 + h); };
+  auto noise_count = [&](int b) { return group(5 * bounces + 6 + 2 * b); };
+  auto noise_head = [&](int b) { return group(5 * bounces + 7 + 2 * b); };
   const dim3 pix_block(16, 16);
   const dim3 pix_grid((p.width + 15) / 16, (p.height + 15) / 16);
   auto mark = [&](int stage, int begin) {
@@ -337,9 +371,11 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
       int32_t* sc = b > 0 ? shadow_count(b - 1) : shadow_count(bounces);
       mark(kStageTrace, 1);
       hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
-                         head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0);
+                         head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b));
       mark(kStageTrace, 0);
-      mark(kStageShade, 1);
+      mark(kStageShade, 1);  // the noise normal maps count as shading
+      hipLaunchKernelGGL(wf_normals<M>, dim3(g_normals), dim3(kThreads), 0, stream, s, c, k, p, w, noise_count(b),
+                         noise_head(b));
       hipLaunchKernelGGL(wf_shade<M>, dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
                          live(b), head(3 * b + 2), w.queue[(b + 1) & 1], live(b + 1), shadow_count(b), b);
       mark(kStageShade, 0);
@@ -350,7 +386,7 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
     mark(kStageEscape, 1);
     hipLaunchKernelGGL((wf_trace<M, true>), dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
                        shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], live(bounces),
-                       head(3 * bounces + 1), 0);
+                       head(3 * bounces + 1), 0, nullptr);
     mark(kStageEscape, 0);
     mark(kStageFilm, 1);
     hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w,
