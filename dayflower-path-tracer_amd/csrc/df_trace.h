@@ -41,9 +41,11 @@ struct Tracer {
   const DSky& K;
   const DParams& P;
   Noise noise;
-  // Noise normal map values precomputed per path (wavefront shading): when
-  // noise_in is set, normal_map_noise reads noise_in[noise_index]; with
-  // noise_skip it leaves the normal unperturbed (the pass computing them).
+  // Values precomputed per path (wavefront shading): when noise_in is set,
+  // a hit on a surface with noise normals takes its three noise values
+  // (x, y, z) and its albedo texture's colour (w, int bits) from
+  // noise_in[noise_index]; with noise_skip, normal_map_noise leaves the
+  // normal unperturbed (the pass computing the values).
   const float4* noise_in = nullptr;
   int noise_index = 0;
   bool noise_skip = false;
@@ -533,15 +535,20 @@ struct Tracer {
       } else {
         noise_normal_values(scale, n);
       }
-      const float nx = n[0], ny = n[1], nz = n[2];
-      const float s1X = h.ns[0] + nx * amount;
-      const float s1Y = h.ns[1] + ny * amount;
-      const float s1Z = h.ns[2] + nz * amount;
-      const float r = jrsqrt(s1X * s1X + s1Y * s1Y + s1Z * s1Z);
-      h.ns[0] = s1X * r;
-      h.ns[1] = s1Y * r;
-      h.ns[2] = s1Z * r;
+      apply_noise_normal(amount, n);
     }
+  }
+
+  // The shading normal perturbed by the noise values n with weight amount.
+  __device__ __forceinline__ void apply_noise_normal(float amount, const float n[3]) {
+    const float nx = n[0], ny = n[1], nz = n[2];
+    const float s1X = h.ns[0] + nx * amount;
+    const float s1Y = h.ns[1] + ny * amount;
+    const float s1Z = h.ns[2] + nz * amount;
+    const float r = jrsqrt(s1X * s1X + s1Y * s1Y + s1Z * s1Z);
+    h.ns[0] = s1X * r;
+    h.ns[1] = s1Y * r;
+    h.ns[2] = s1Z * r;
   }
 
   __device__ __forceinline__ void normal_map_image(const DPrim& pr) {  // :2714-2846
@@ -972,7 +979,10 @@ struct Tracer {
       aG = P.diff_albedo[aslot * 3 + 1];
       aB = P.diff_albedo[aslot * 3 + 2];
     } else {
-      const int alb = tex_color(jf2i(S.surfaces[so + 1]));
+      // the albedo texture of a surface with noise normals: evaluated by the
+      // pass that computed the noise values, at this hit record
+      const int alb = noise_in && has_noise_normals(po / 3) ? __float_as_int(noise_in[noise_index].w)
+                                                            : tex_color(jf2i(S.surfaces[so + 1]));
       aR = ch_r(alb);
       aG = ch_g(alb);
       aB = ch_b(alb);

@@ -197,9 +197,14 @@ __global__ __launch_bounds__(kThreads) void wf_normals(DScene S, DCamera C, DSky
     const float4 o = W.ray_o[i], d = W.ray_d[i], hv = W.hit[i];
     const int ci = __float_as_int(hv.y);
     tr.fill_hit(o.x, o.y, o.z, d.x, d.y, d.z, hv.x, ci, __float_as_int(hv.z), __float_as_int(hv.w));
+    const int so = S.prims[ci].surface_offset;
     float n[3];
-    tr.noise_normal_values(S.surfaces[S.prims[ci].surface_offset + 5], n);
-    W.noise[i] = make_float4(n[0], n[1], n[2], 0.0f);
+    tr.noise_normal_values(S.surfaces[so + 5], n);
+    // the hit record as the shade kernel's fill_hit leaves it, then the
+    // albedo texture shade would evaluate on it
+    tr.apply_noise_normal(S.surfaces[so + 4], n);
+    const int alb = tr.tex_color(jf2i(S.surfaces[so + 1]));
+    W.noise[i] = make_float4(n[0], n[1], n[2], __int_as_float(alb));
   });
 }
 
