@@ -55,8 +55,8 @@ KERNEL_SYMBOLS = {"megakernel": "dfk::render_kernel<0, 2>", "raygen": "dfk::wf_r
                   "trace": "dfk::wf_trace<0, false>", "shade": "dfk::wf_shade<0>", "escape": "dfk::wf_trace<0, true>",
                   "film": "dfk::wf_film<0>"}
 KERNEL_NAMES = {"megakernel": "render_kernel (one work-item per pixel)", "raygen": "wf_raygen",
-                "trace": "wf_trace<0,false> (a bounce's sun any-hit + closest-hit queries)", "shade": "wf_shade",
-                "escape": "wf_trace<0,true> (escape tests)", "film": "wf_film"}
+                "trace": "wf_trace<0,false> (a bounce's closest-hit + sun any-hit queries)", "shade": "wf_shade",
+                "escape": "wf_trace<0,true> (escape tests + the last sun queries)", "film": "wf_film"}
 PEAK_HBM_GBS = 8000.0
 
 
