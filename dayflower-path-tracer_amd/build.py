@@ -8,6 +8,7 @@ Usage: python dayflower-path-tracer_amd/build.py [--jobs N]
 """
 
 import argparse
+import hashlib
 import os
 import subprocess
 import sys
@@ -66,12 +67,41 @@ def build_hip_lib(jobs):
     return out
 
 
+STAMP_PATH = os.path.join(LIB_DIR, "sources.stamp")
+
+
+def source_hash():
+    """SHA-256 over the names and contents of every source and header the two libraries are built from."""
+    h = hashlib.sha256()
+    roots = [CSRC, os.path.join(os.path.dirname(HERE), "include")]
+    for root in roots:
+        for dirpath, _, files in sorted(os.walk(root)):
+            for f in sorted(files):
+                if f.endswith((".hip", ".cpp", ".h", ".hpp")):
+                    path = os.path.join(dirpath, f)
+                    h.update(os.path.relpath(path, os.path.dirname(HERE)).encode())
+                    h.update(open(path, "rb").read())
+    return h.hexdigest()
+
+
+def is_current():
+    """True when both libraries exist and were built from the sources as they are now."""
+    libs = [os.path.join(LIB_DIR, n) for n in ("libdayflower_scene.so", "libdayflower_hip.so")]
+    if not all(os.path.exists(p) for p in libs) or not os.path.exists(STAMP_PATH):
+        return False
+    return open(STAMP_PATH).read().strip() == source_hash()
+
+
 def build(jobs=4):
     os.makedirs(LIB_DIR, exist_ok=True)
+    digest = source_hash()
     with ThreadPoolExecutor(max_workers=2) as ex:
         f1 = ex.submit(build_scene_lib)
         f2 = ex.submit(build_hip_lib, jobs)
-        return [f1.result(), f2.result()]
+        out = [f1.result(), f2.result()]
+    with open(STAMP_PATH, "w") as fh:
+        fh.write(digest + "\n")
+    return out
 
 
 if __name__ == "__main__":

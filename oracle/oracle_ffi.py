@@ -7,6 +7,7 @@ baseline; the product never imports it.
 """
 
 import ctypes
+import hashlib
 import os
 import subprocess
 
@@ -14,6 +15,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+STAMP_PATH = os.path.join(HERE, "liboracle.stamp")
+_SOURCES = [os.path.join(HERE, "df_oracle.cpp"), os.path.join(HERE, "Makefile"),
+            os.path.join(os.path.dirname(HERE), "include", "dayflower_types.h")]
 
 _c_float_p = ctypes.POINTER(ctypes.c_float)
 
@@ -36,9 +40,27 @@ class OracleCounters(ctypes.Structure):
                                                "sphere_tests", "terrain_steps", "sky_evals")]
 
 
+def source_hash():
+    h = hashlib.sha256()
+    for path in _SOURCES:
+        h.update(os.path.basename(path).encode())
+        h.update(open(path, "rb").read())
+    return h.hexdigest()
+
+
+def is_current():
+    """True when liboracle.so exists and was built from the sources as they are now."""
+    if not (os.path.exists(LIB_PATH) and os.path.exists(STAMP_PATH)):
+        return False
+    return open(STAMP_PATH).read().strip() == source_hash()
+
+
 def build():
-    """Compiles liboracle.so from df_oracle.cpp (make)."""
-    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    """Compiles liboracle.so from df_oracle.cpp (make, always rebuilding) and stamps the sources' hash."""
+    digest = source_hash()
+    subprocess.run(["make", "-s", "-B", "-C", HERE], check=True)
+    with open(STAMP_PATH, "w") as fh:
+        fh.write(digest + "\n")
     return LIB_PATH
 
 

@@ -15,19 +15,19 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Builds the libraries only when nothing is built yet."""
-    from dayflower_amd import _ffi
+    """Builds a library when it is missing or was built from other sources than the tree holds now (each build
+    stamps the SHA-256 of its sources; a built .so travels with the tree, so an edited source must never be
+    tested through the binary of an older one)."""
+    import importlib.util
 
-    if not (os.path.exists(_ffi.HIP_LIB_PATH) and os.path.exists(_ffi.SCENE_LIB_PATH)):
-        import importlib.util
-
-        spec = importlib.util.spec_from_file_location("dayflower_build", os.path.join(PKG_DIR, "build.py"))
-        mod = importlib.util.module_from_spec(spec)
-        spec.loader.exec_module(mod)
+    spec = importlib.util.spec_from_file_location("dayflower_build", os.path.join(PKG_DIR, "build.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    if not mod.is_current():
         mod.build(jobs=4)
     from oracle import oracle_ffi
 
-    if not os.path.exists(oracle_ffi.LIB_PATH):
+    if not oracle_ffi.is_current():
         oracle_ffi.build()
 
 
