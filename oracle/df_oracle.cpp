@@ -25,11 +25,41 @@
 #include <cstdint>
 #include <cstring>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../include/dayflower_types.h"
 
 namespace {
+
+// ------------------------------------------------------------ planted mistakes
+// The pin tests switch on one deliberate, plausible mistake at a time (oracle_plant) to show that each of
+// their checks fails on it.  0 is the faithful restatement; nothing but those tests sets another value.
+enum Plant {
+  PLANT_NONE = 0,
+  PLANT_CAMERA_RES = 1,        // pixel coordinates normalised by res instead of res - 1
+  PLANT_CAMERA_BOX = 2,        // box filter in place of the tent
+  PLANT_LENS_RADIUS = 3,       // lens radius aperture * r instead of aperture * sqrt(r)
+  PLANT_FISHEYE_W = 4,         // fisheye w = 1 - r^2 instead of sqrt(1 - r^2)
+  PLANT_LAMBERT_UNIFORM = 5,   // uniform hemisphere instead of cosine-weighted
+  PLANT_PHONG_EXPONENT = 6,    // Phong lobe exponent 20 instead of 50
+  PLANT_FRESNEL_PROB = 7,      // reflection probability R instead of 0.25 + 0.5 R
+  PLANT_SNELL_RATIO = 8,       // index ratio inverted
+  PLANT_SUN_CONE = 9,          // sin^2 of the half-angle r / d^2 instead of r^2 / d^2
+  PLANT_AO_UNIFORM = 10,       // AO rays uniform over the hemisphere
+  PLANT_PHONG_SHADER = 11,     // ray caster's specular exponent a tenth of 250
+  PLANT_SPHERE_UV = 12,        // atan2 operands swapped
+  PLANT_BARYCENTRIC = 13,      // barycentric weights of B and C swapped
+  PLANT_NORMAL_MAP_B = 14,     // normal map blue decoded like red and green
+  PLANT_NOISE_NORMAL = 15,     // noise normal's y component from the z rotation of the point
+  PLANT_BULLSEYE = 16,         // bullseye ring frequency 0.5 instead of 0.25
+  PLANT_PEREZ_THETA0 = 17,     // chromaticity denominators at the sun's zenith angle (no theta-1 quirk)
+  PLANT_PEREZ_SWAP = 18,       // luminance Perez coefficients D and E exchanged
+  PLANT_FBM_TEXTURE = 19,      // fBm texture noise mapped to [-1, 1] instead of [0, 1]
+  PLANT_CLEARCOAT_OFFSET = 20  // ClearCoat reflection leaves at 0.01 instead of 0.02 along the normal
+};
+int g_plant = PLANT_NONE;
+inline bool planted(int k) { return g_plant == k; }
 
 // ------------------------------------------------------------ Java semantics
 
@@ -445,6 +475,14 @@ class Item {
   }
 
   void run();  // GPURendererKernel.run (:252-308)
+  // run() with the primary ray given instead of doCreatePrimaryRay's
+  void run_ray(const float ray[6]);
+  // doCreatePrimaryRay (:573-724): false for no ray (fisheye, outside the unit disk)
+  bool primary_ray(float ray[6]) {
+    const bool ok = create_primary_ray();
+    std::memcpy(ray, ray_, sizeof ray_);
+    return ok;
+  }
 
   // exposed for unit tests
   float intersect(float ox, float oy, float oz, float dx, float dy, float dz, bool testing) {
@@ -597,8 +635,29 @@ class Item {
                     float kdb, float ksr, float ksg, float ksb, float ns);
   void do_sky(bool active, float dx, float dy, float dz);
   void do_sun(bool active, float px, float py, float pz, float nx, float ny, float nz, float ar, float ag, float ab);
+
+ public:
+  void sun_direction(float pX, float pY, float pZ, float u, float v, float d[3]) const;
+
+ private:
   void render_ao();
   void render_pt(bool sky_active, bool sun_active);
+
+ public:
+  void scatter(int mat, int aslot, float pX, float pY, float pZ, float sX, float sY, float sZ, float wX, float wY,
+               float wZ, float dp2, float aR, float aG, float aB, float& ox, float& oy, float& oz, float& dx,
+               float& dy, float& dz, float& mr, float& mg, float& mb);
+  // the oriented normal w (s flipped to face the incoming direction d) and dp2 = 2 s.d (:3219-3225)
+  static void orient(float sX, float sY, float sZ, float dx, float dy, float dz, float* w, float* dp2) {
+    const float dp = sX * dx + sY * dy + sZ * dz;
+    *dp2 = dp * 2.0f;
+    const bool ok = dp < 0.0f;
+    w[0] = ok ? sX : -sX;
+    w[1] = ok ? sY : -sY;
+    w[2] = ok ? sZ : -sZ;
+  }
+
+ private:
   void tan_sky(bool active, float dx, float dy, float dz);
   void tan_vertex(int aslot, int eslot, const float a[3], const float e[3], bool sun_term, float pX, float pY,
                   float pZ, float wX, float wY, float wZ);
@@ -607,6 +666,7 @@ class Item {
   void render_rt(bool sky_active);
   void render_sn(bool sky_active);
   void render_wireframes();
+  void shade_primary();
   float get_y(float x, float z) const {
     const df_params* p = s_.p;
     return simplex_fractal_xy(p->global_amplitude, p->global_frequency, p->global_gain, p->global_lacunarity,
@@ -723,8 +783,13 @@ bool Item::create_primary_ray() {  // GPURendererKernel.java:573-724
     sampleY = next_float();
     sampleX = sampleX < 0.5f ? jsqrt(2.0f * sampleX) - 1.0f : 1.0f - jsqrt(2.0f - 2.0f * sampleX);
     sampleY = sampleY < 0.5f ? jsqrt(2.0f * sampleY) - 1.0f : 1.0f - jsqrt(2.0f - 2.0f * sampleY);
+    if (planted(PLANT_CAMERA_BOX)) {
+      sampleX = jsqrt(2.0f * sampleX + 2.0f) * 0.5f - 0.5f;  // any non-tent jitter in [-0.5, 0.5]
+      sampleY = 0.0f;
+    }
   }
-  const float sx = (sampleX + static_cast<float>(x)) / (c[16] - 1.0f);
+  const float rx = planted(PLANT_CAMERA_RES) ? c[16] : c[16] - 1.0f;
+  const float sx = (sampleX + static_cast<float>(x)) / rx;
   const float sy = (sampleY + static_cast<float>(y)) / (c[17] - 1.0f);
   const float sx0 = 2.0f * sx - 1.0f;
   const float sy0 = 2.0f * sy - 1.0f;
@@ -732,7 +797,7 @@ bool Item::create_primary_ray() {  // GPURendererKernel.java:573-724
   if (c[18] == 2.0f) {  // CAMERA_LENS_FISHEYE
     const float dp = sx0 * sx0 + sy0 * sy0;
     if (dp > 1.0f) return false;
-    w = jsqrt(1.0f - dp);
+    w = planted(PLANT_FISHEYE_W) ? 1.0f - dp : jsqrt(1.0f - dp);
   }
   const float mX = eyeX + wX * w, mY = eyeY + wY * w, mZ = eyeZ + wZ * w;
   const float ppX = mX + (hX * sx0) + (vtX * sy0);
@@ -748,7 +813,7 @@ bool Item::create_primary_ray() {  // GPURendererKernel.java:573-724
     const float r1 = next_float();
     const float r2 = next_float();
     const float angle = PI_MULTIPLIED_BY_TWO * r1;
-    const float dist = aperture * jsqrt(r2);
+    const float dist = planted(PLANT_LENS_RADIUS) ? aperture * r2 : aperture * jsqrt(r2);
     const float ax = jcos(angle) * dist;
     const float ay = jsin(angle) * dist;
     apX = eyeX + uX * ax + vX * ay;
@@ -1034,7 +1099,7 @@ void Item::props_sphere(float ox, float oy, float oz, float dx, float dy, float 
   const float n0X = pX - pos[0], n0Y = pY - pos[1], n0Z = pZ - pos[2];
   const float r = jrsqrt(n0X * n0X + n0Y * n0Y + n0Z * n0Z);
   const float nX = n0X * r, nY = n0Y * r, nZ = n0Z * r;
-  const float u = 0.5f + jatan2(-nZ, -nX) * PI_MULTIPLIED_BY_TWO_RECIPROCAL;
+  const float u = 0.5f + (planted(PLANT_SPHERE_UV) ? jatan2(-nX, -nZ) : jatan2(-nZ, -nX)) * PI_MULTIPLIED_BY_TWO_RECIPROCAL;
   const float v = 0.5f - jasinpi(-nY);
   rec_[R_P] = pX; rec_[R_P + 1] = pY; rec_[R_P + 2] = pZ;
   rec_[R_NG] = nX; rec_[R_NG + 1] = nY; rec_[R_NG + 2] = nZ;
@@ -1079,9 +1144,10 @@ void Item::props_triangle(float ox, float oy, float oz, float dx, float dy, floa
   const float v2X = v1Y * abZ - v1Z * abY, v2Y = v1Z * abX - v1X * abZ, v2Z = v1X * abY - v1Y * abX;
   const float det = abX * v0X + abY * v0Y + abZ * v0Z;
   const float dr = 1.0f / det;
-  const float bu = (v1X * v0X + v1Y * v0Y + v1Z * v0Z) * dr;
-  const float bv = (dx * v2X + dy * v2Y + dz * v2Z) * dr;
+  float bu = (v1X * v0X + v1Y * v0Y + v1Z * v0Z) * dr;
+  float bv = (dx * v2X + dy * v2Y + dz * v2Z) * dr;
   const float bw = 1.0f - bu - bv;
+  if (planted(PLANT_BARYCENTRIC)) std::swap(bu, bv);
   const float u = bw * uvA[0] + bu * uvB[0] + bv * uvC[0];
   const float v = bw * uvA[1] + bu * uvB[1] + bv * uvC[1];
   rec_[R_T] = t;
@@ -1136,7 +1202,8 @@ void Item::normal_map_noise(int po) {  // :2658-2712
     const float sr = 1.0f / scale;
     const float x1 = x0 * sr, y1 = y0 * sr, z1 = z0 * sr;
     const float nx = simplex_fbm_xyz(scale, 0.5f, -0.26f, 0.26f, 16, x1, y1, z1);
-    const float ny = simplex_fbm_xyz(scale, 0.5f, -0.26f, 0.26f, 16, y1, z1, x1);
+    const float ny = planted(PLANT_NOISE_NORMAL) ? simplex_fbm_xyz(scale, 0.5f, -0.26f, 0.26f, 16, z1, x1, y1)
+                                                 : simplex_fbm_xyz(scale, 0.5f, -0.26f, 0.26f, 16, y1, z1, x1);
     const float nz = simplex_fbm_xyz(scale, 0.5f, -0.26f, 0.26f, 16, z1, x1, y1);
     const float s1X = rec_[R_NS] + nx * amount;
     const float s1Y = rec_[R_NS + 1] + ny * amount;
@@ -1156,7 +1223,7 @@ void Item::normal_map_image(int po) {  // :2714-2846
   const int rgb = tex_image_simple(to);
   const float r = ((((rgb >> 16) & 0xFF) * COLOR_RECIPROCAL) - 0.5f) * 2.0f;
   const float g = ((((rgb >> 8) & 0xFF) * COLOR_RECIPROCAL) - 0.5f) * 2.0f;
-  const float b = ((((rgb >> 0) & 0xFF) * COLOR_RECIPROCAL) - 0.0f) * 2.0f;
+  const float b = ((((rgb >> 0) & 0xFF) * COLOR_RECIPROCAL) - (planted(PLANT_NORMAL_MAP_B) ? 0.5f : 0.0f)) * 2.0f;
   const float wX = rec_[R_NS], wY = rec_[R_NS + 1], wZ = rec_[R_NS + 2];
   const int type = s_.a->primitives[po];
   float sX, sY, sZ;
@@ -1263,7 +1330,7 @@ int Item::tex_bullseye(int off) {  // :1668-1712
   const float bR = ((b >> 16) & 0xFF) * COLOR_RECIPROCAL, bG = ((b >> 8) & 0xFF) * COLOR_RECIPROCAL,
               bB = (b & 0xFF) * COLOR_RECIPROCAL;
   const float length = jsqrt(x * x + y * y + z * z);
-  const float value = remainder(length * 0.25f, 1.0f);
+  const float value = remainder(length * (planted(PLANT_BULLSEYE) ? 0.5f : 0.25f), 1.0f);
   float cR = value > 0.5f ? aR : bR;
   float cG = value > 0.5f ? aG : bG;
   float cB = value > 0.5f ? aB : bB;
@@ -1315,7 +1382,8 @@ int Item::tex_fbm(int off) {  // :1776-1815
               mB = (mu & 0xFF) * COLOR_RECIPROCAL;
   const float freq = T[off + 4], gain = T[off + 5];
   const int oct = jf2i(T[off + 6]);
-  const float n = simplex_fbm_xyz(freq, gain, 0.0f, 1.0f, oct, x, y, z);
+  const float n = planted(PLANT_FBM_TEXTURE) ? simplex_fbm_xyz(freq, gain, -1.0f, 1.0f, oct, x, y, z)
+                                             : simplex_fbm_xyz(freq, gain, 0.0f, 1.0f, oct, x, y, z);
   return pack_rgb(saturate(n * mR + aR, 0.0f, 1.0f) * 255.0f + 0.5f, saturate(n * mG + aG, 0.0f, 1.0f) * 255.0f + 0.5f,
                   saturate(n * mB + aB, 0.0f, 1.0f) * 255.0f + 0.5f);
 }
@@ -1405,7 +1473,7 @@ int Item::phong_shader0(bool check, float pX, float pY, float pZ, float nX, floa
       const float rX = 2.0f * ldn * nX - lnX, rY = 2.0f * ldn * nY - lnY, rZ = 2.0f * ldn * nZ - lnZ;
       const float rr = jrsqrt(rX * rX + rY * rY + rZ * rZ);
       const float rdv = rX * rr * vX + rY * rr * vY + rZ * rr * vZ;
-      const float sh = jpow(jmaxf(rdv, 0.0f), ns);
+      const float sh = jpow(jmaxf(rdv, 0.0f), planted(PLANT_PHONG_SHADER) ? ns * 0.1f : ns);
       const float is = 1.0f;
       r += ksR * sh * is;
       g += ksG * sh * is;
@@ -1441,13 +1509,15 @@ void Item::do_sky(bool active, float dx, float dy, float dz) {  // :2127-2236
     const double* L = S->perez_relative_luminance;
     const double* X = S->perez_x;
     const double* Y = S->perez_y;
-    const double lden = ((1.0 + L[0] * std::exp(L[1])) * (1.0 + L[2] * std::exp(L[3] * theta0) + L[4] * cosTheta0 * cosTheta0));
-    const double lnum = ((1.0 + L[0] * std::exp(L[1] * cosTheta1R)) * (1.0 + L[2] * std::exp(L[3] * gamma) + L[4] * cosGamma * cosGamma));
+    const double L3 = planted(PLANT_PEREZ_SWAP) ? L[4] : L[3], L4 = planted(PLANT_PEREZ_SWAP) ? L[3] : L[4];
+    const double lden = ((1.0 + L[0] * std::exp(L[1])) * (1.0 + L[2] * std::exp(L3 * theta0) + L4 * cosTheta0 * cosTheta0));
+    const double lnum = ((1.0 + L[0] * std::exp(L[1] * cosTheta1R)) * (1.0 + L[2] * std::exp(L3 * gamma) + L4 * cosGamma * cosGamma));
+    const double thetaD = planted(PLANT_PEREZ_THETA0) ? theta0 : theta1;  // the kernel's quirk: theta1
     const double lum = S->zenith_relative_luminance * lnum / lden * 1.0e-4;
-    const double xden = ((1.0 + X[0] * std::exp(X[1])) * (1.0 + X[2] * std::exp(X[3] * theta1) + X[4] * cosTheta0 * cosTheta0));
+    const double xden = ((1.0 + X[0] * std::exp(X[1])) * (1.0 + X[2] * std::exp(X[3] * thetaD) + X[4] * cosTheta0 * cosTheta0));
     const double xnum = ((1.0 + X[0] * std::exp(X[1] * cosTheta1R)) * (1.0 + X[2] * std::exp(X[3] * gamma) + X[4] * cosGamma * cosGamma));
     const double x = S->zenith_x * xnum / xden;
-    const double yden = ((1.0 + Y[0] * std::exp(Y[1])) * (1.0 + Y[2] * std::exp(Y[3] * theta1) + Y[4] * cosTheta0 * cosTheta0));
+    const double yden = ((1.0 + Y[0] * std::exp(Y[1])) * (1.0 + Y[2] * std::exp(Y[3] * thetaD) + Y[4] * cosTheta0 * cosTheta0));
     const double ynum = ((1.0 + Y[0] * std::exp(Y[1] * cosTheta1R)) * (1.0 + Y[2] * std::exp(Y[3] * gamma) + Y[4] * cosGamma * cosGamma));
     const double y = S->zenith_y * ynum / yden;
     daylight_xyy_to_srgb(static_cast<float>(x), static_cast<float>(y), lum, &r, &g, &b);
@@ -1457,6 +1527,41 @@ void Item::do_sky(bool active, float dx, float dy, float dz) {  // :2127-2236
   tmp_[2] = b;
 }
 
+// The direction towards the sun that doCalculateColorForSun samples from the point p with the random numbers
+// (u, v) (GPURendererKernel.java:2252-2320): uniform over the cone about the sun direction whose half-angle is
+// that of a disk of radius 500 seen from p at the sun's origin.
+void Item::sun_direction(float pX, float pY, float pZ, float u, float v, float d[3]) const {
+  const df_sky* S = s_.sky;
+  const float sdX = S->sun_direction_world[0], sdY = S->sun_direction_world[1], sdZ = S->sun_direction_world[2];
+  const float sunR = 500.0f;
+  const float sunR2 = sunR * sunR;
+  const float toX = S->sun_origin[0] - pX, toY = S->sun_origin[1] - pY, toZ = S->sun_origin[2] - pZ;
+  const float to2 = toX * toX + toY * toY + toZ * toZ;
+  const float sinMax2 = planted(PLANT_SUN_CONE) ? sunR / to2 : sunR2 / to2;
+  const float cosMax = jsqrt(jmaxf(0.0f, 1.0f - sinMax2));
+  const float cosT = u * (cosMax - 1.0f) + 1.0f;
+  const float sinT = jsqrt(jmaxf(0.0f, 1.0f - cosT * cosT));
+  const float phi = PI_MULTIPLIED_BY_TWO * v;
+  const float x = jcos(phi) * sinT, y = jsin(phi) * sinT, z = cosT;
+  const float wX = sdX, wY = sdY, wZ = sdZ;
+  const float aX = jabs(wX), aY = jabs(wY), aZ = jabs(wZ);
+  const bool c0 = aX < aY && aX < aZ;
+  const bool c1 = aY < aZ;
+  const float vX = c0 ? 0.0f : c1 ? wZ : wY;
+  const float vY = c0 ? wZ : c1 ? 0.0f : -wX;
+  const float vZ = c0 ? -wY : c1 ? -wX : 0.0f;
+  const float vr = jrsqrt(vX * vX + vY * vY + vZ * vZ);
+  const float vnX = vX * vr, vnY = vY * vr, vnZ = vZ * vr;
+  const float unX = vnY * wZ - vnZ * wY, unY = vnZ * wX - vnX * wZ, unZ = vnX * wY - vnY * wX;
+  const float rX = unX * x + vnX * y + wX * z;
+  const float rY = unY * x + vnY * y + wY * z;
+  const float rZ = unZ * x + vnZ * y + wZ * z;
+  const float rr = jrsqrt(rX * rX + rY * rY + rZ * rZ);
+  d[0] = rX * rr;
+  d[1] = rY * rr;
+  d[2] = rZ * rr;
+}
+
 void Item::do_sun(bool active, float pX, float pY, float pZ, float nX, float nY, float nZ, float aR, float aG,
                   float aB) {  // :2238-2347
   float r = 0.0f, g = 0.0f, b = 0.0f;
@@ -1464,35 +1569,13 @@ void Item::do_sun(bool active, float pX, float pY, float pZ, float nX, float nY,
   if (active) {
     const df_sky* S = s_.sky;
     const float sdX = S->sun_direction_world[0], sdY = S->sun_direction_world[1], sdZ = S->sun_direction_world[2];
-    const float sunR = 500.0f;
-    const float sunR2 = sunR * sunR;
-    const float toX = S->sun_origin[0] - pX, toY = S->sun_origin[1] - pY, toZ = S->sun_origin[2] - pZ;
-    const float to2 = toX * toX + toY * toY + toZ * toZ;
     const float u = next_float();
     const float v = next_float();
     sun_u_ = u;
     sun_v_ = v;
-    const float sinMax2 = sunR2 / to2;
-    const float cosMax = jsqrt(jmaxf(0.0f, 1.0f - sinMax2));
-    const float cosT = u * (cosMax - 1.0f) + 1.0f;
-    const float sinT = jsqrt(jmaxf(0.0f, 1.0f - cosT * cosT));
-    const float phi = PI_MULTIPLIED_BY_TWO * v;
-    const float x = jcos(phi) * sinT, y = jsin(phi) * sinT, z = cosT;
-    const float wX = sdX, wY = sdY, wZ = sdZ;
-    const float aX = jabs(wX), aY = jabs(wY), aZ = jabs(wZ);
-    const bool c0 = aX < aY && aX < aZ;
-    const bool c1 = aY < aZ;
-    const float vX = c0 ? 0.0f : c1 ? wZ : wY;
-    const float vY = c0 ? wZ : c1 ? 0.0f : -wX;
-    const float vZ = c0 ? -wY : c1 ? -wX : 0.0f;
-    const float vr = jrsqrt(vX * vX + vY * vY + vZ * vZ);
-    const float vnX = vX * vr, vnY = vY * vr, vnZ = vZ * vr;
-    const float unX = vnY * wZ - vnZ * wY, unY = vnZ * wX - vnX * wZ, unZ = vnX * wY - vnY * wX;
-    const float rX = unX * x + vnX * y + wX * z;
-    const float rY = unY * x + vnY * y + wY * z;
-    const float rZ = unZ * x + vnZ * y + wZ * z;
-    const float rr = jrsqrt(rX * rX + rY * rY + rZ * rZ);
-    const float dX = rX * rr, dY = rY * rr, dZ = rZ * rr;
+    float d3[3];
+    sun_direction(pX, pY, pZ, u, v, d3);
+    const float dX = d3[0], dY = d3[1], dZ = d3[2];
     const float dp0 = dX * sdX + dY * sdY + dZ * sdZ;
     const float dp1 = dX * nX + dY * nY + dZ * nZ;
     if (dp0 > 0.0f && dp1 > 0.0f) {
@@ -1541,7 +1624,8 @@ void Item::render_ao() {  // :3029-3149
     const float xj = next_float();
     const float phi = PI_MULTIPLIED_BY_TWO * xi;
     const float cp = jcos(phi), sp = jsin(phi);
-    const float st = jsqrt(xj), ct = jsqrt(1.0f - xj);
+    const float ct = planted(PLANT_AO_UNIFORM) ? 1.0f - xj : jsqrt(1.0f - xj);
+    const float st = planted(PLANT_AO_UNIFORM) ? jsqrt(1.0f - ct * ct) : jsqrt(xj);
     const float d0X = cp * st, d0Y = sp * st, d0Z = ct;
     const float d1X = d0X * u2X + d0Y * v0X + d0Z * w1X;
     const float d1Y = d0X * u2Y + d0Y * v0Y + d0Z * w1Y;
@@ -1589,10 +1673,9 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
     const int so = A->primitives[po + 2];
     const float pX = rec_[R_P], pY = rec_[R_P + 1], pZ = rec_[R_P + 2];
     const float sX = rec_[R_NS], sY = rec_[R_NS + 1], sZ = rec_[R_NS + 2];
-    const float dp = sX * dx + sY * dy + sZ * dz;
-    const float dp2 = dp * 2.0f;
-    const bool ok = dp < 0.0f;
-    const float wX = ok ? sX : -sX, wY = ok ? sY : -sY, wZ = ok ? sZ : -sZ;
+    float w3[3], dp2;
+    orient(sX, sY, sZ, dx, dy, dz, w3, &dp2);
+    const float wX = w3[0], wY = w3[1], wZ = w3[2];
     const int ta = jf2i(A->surfaces[so + 1]);
     const int te = jf2i(A->surfaces[so + 2]);
     const int alb = tex_color(ta);
@@ -1647,195 +1730,7 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
       aG *= pdfr;
       aB *= pdfr;
     }
-    if (mat == 1) {  // ClearCoat :3323-3464
-      const float a = 1.5f - 1.0f, b = 1.5f + 1.0f, c = 1.0f / 1.5f;
-      const float e = (a * a) / (b * b);
-      const float nnt = c;
-      const float dp0 = wX * dx + wY * dy + wZ * dz;
-      const float tir = 1.0f - nnt * nnt * (1.0f - dp0 * dp0);
-      const float rX = dx - sX * dp2, rY = dy - sY * dp2, rZ = dz - sZ * dp2;
-      if (tir < 0.0f) {
-        ox = pX + wX * 0.02f;
-        oy = pY + wY * 0.02f;
-        oz = pZ + wZ * 0.02f;
-        dx = rX;
-        dy = rY;
-        dz = rZ;
-      } else {
-        const float a1 = -dp0;
-        const float a2 = 1.0f - a1;
-        const float refl = e + (1.0f - e) * a2 * a2 * a2 * a2 * a2;
-        const float trans = 1.0f - refl;
-        const float prob = 0.25f + 0.5f * refl;
-        const float rp = refl / prob;
-        const float tp = trans / (1.0f - prob);
-        const bool isR = next_float() < prob;
-        const float m = isR ? rp : tp;
-        mr *= m;
-        mg *= m;
-        mb *= m;
-        if (tan_) tan_->scale(m);
-        if (isR) {
-          ox = pX + wX * 0.02f;
-          oy = pY + wY * 0.02f;
-          oz = pZ + wZ * 0.02f;
-          dx = rX;
-          dy = rY;
-          dz = rZ;
-        } else {
-          const float u = next_float();
-          const float v = next_float();
-          const float phi = PI_MULTIPLIED_BY_TWO * u;
-          const float ct = jsqrt(v), st = jsqrt(1.0f - v);
-          const float x = jcos(phi) * st, y = jsin(phi) * st, z = ct;
-          const bool isY = jabs(wX) > 0.1f;
-          const float uX = (isY ? 1.0f : 0.0f) * wZ;
-          const float uY = -((isY ? 0.0f : 1.0f) * wZ);
-          const float uZ = (isY ? 0.0f : 1.0f) * wY - (isY ? 1.0f : 0.0f) * wX;
-          const float ur = jrsqrt(uX * uX + uY * uY + uZ * uZ);
-          const float unX = uX * ur, unY = uY * ur, unZ = uZ * ur;
-          const float vnX = wY * unZ - wZ * unY, vnY = wZ * unX - wX * unZ, vnZ = wX * unY - wY * unX;
-          const float lX = unX * x + vnX * y + wX * z;
-          const float lY = unY * x + vnY * y + wY * z;
-          const float lZ = unZ * x + vnZ * y + wZ * z;
-          const float lr = jrsqrt(lX * lX + lY * lY + lZ * lZ);
-          ox = pX + wX * 0.01f;
-          oy = pY + wY * 0.01f;
-          oz = pZ + wZ * 0.01f;
-          dx = lX * lr;
-          dy = lY * lr;
-          dz = lZ * lr;
-          mr *= aR;
-          mg *= aG;
-          mb *= aB;
-          if (tan_) {
-            const float a3[3] = {aR, aG, aB};
-            tan_->albedo(aslot, a3);
-          }
-        }
-      }
-    } else if (mat == 3) {  // Lambertian :3465-3515
-      const float u = next_float();
-      const float v = next_float();
-      const float phi = PI_MULTIPLIED_BY_TWO * u;
-      const float ct = jsqrt(v), st = jsqrt(1.0f - v);
-      const float x = jcos(phi) * st, y = jsin(phi) * st, z = ct;
-      const bool isY = jabs(wX) > 0.1f;
-      const float uX = (isY ? 1.0f : 0.0f) * wZ;
-      const float uY = -((isY ? 0.0f : 1.0f) * wZ);
-      const float uZ = (isY ? 0.0f : 1.0f) * wY - (isY ? 1.0f : 0.0f) * wX;
-      const float ur = jrsqrt(uX * uX + uY * uY + uZ * uZ);
-      const float unX = uX * ur, unY = uY * ur, unZ = uZ * ur;
-      const float vnX = wY * unZ - wZ * unY, vnY = wZ * unX - wX * unZ, vnZ = wX * unY - wY * unX;
-      const float lX = unX * x + vnX * y + wX * z;
-      const float lY = unY * x + vnY * y + wY * z;
-      const float lZ = unZ * x + vnZ * y + wZ * z;
-      const float lr = jrsqrt(lX * lX + lY * lY + lZ * lZ);
-      ox = pX + wX * 0.01f;
-      oy = pY + wY * 0.01f;
-      oz = pZ + wZ * 0.01f;
-      dx = lX * lr;
-      dy = lY * lr;
-      dz = lZ * lr;
-      mr *= aR;
-      mg *= aG;
-      mb *= aB;
-      if (tan_) {
-        const float a3[3] = {aR, aG, aB};
-        tan_->albedo(aslot, a3);
-      }
-    } else if (mat == 4) {  // Phong :3516-3576
-      const float exponent = 50.0f;
-      const float u = next_float();
-      const float v = next_float();
-      const float phi = PI_MULTIPLIED_BY_TWO * u;
-      const float ct = jpow(1.0f - v, 1.0f / (exponent + 1.0f));
-      const float st = jsqrt(jmaxf(0.0f, 1.0f - ct * ct));
-      const float x = jcos(phi) * st, y = jsin(phi) * st, z = ct;
-      const float rwX = dx - sX * dp2, rwY = dy - sY * dp2, rwZ = dz - sZ * dp2;
-      const float rwr = jrsqrt(rwX * rwX + rwY * rwY + rwZ * rwZ);
-      const float rnX = rwX * rwr, rnY = rwY * rwr, rnZ = rwZ * rwr;
-      const bool isY = jabs(rnX) > 0.1f;
-      const float uX = (isY ? 1.0f : 0.0f) * rnZ;
-      const float uY = -((isY ? 0.0f : 1.0f) * rnZ);
-      const float uZ = (isY ? 0.0f : 1.0f) * rnY - (isY ? 1.0f : 0.0f) * rnX;
-      const float ur = jrsqrt(uX * uX + uY * uY + uZ * uZ);
-      const float unX = uX * ur, unY = uY * ur, unZ = uZ * ur;
-      const float vnX = rnY * unZ - rnZ * unY, vnY = rnZ * unX - rnX * unZ, vnZ = rnX * unY - rnY * unX;
-      const float qX = unX * x + vnX * y + rwX * z;
-      const float qY = unY * x + vnY * y + rwY * z;
-      const float qZ = unZ * x + vnZ * y + rwZ * z;
-      const float qr = jrsqrt(qX * qX + qY * qY + qZ * qZ);
-      ox = pX + rnX * 0.01f;
-      oy = pY + rnY * 0.01f;
-      oz = pZ + rnZ * 0.01f;
-      dx = qX * qr;
-      dy = qY * qr;
-      dz = qZ * qr;
-      mr *= aR;
-      mg *= aG;
-      mb *= aB;
-      if (tan_) {
-        const float a3[3] = {aR, aG, aB};
-        tan_->albedo(aslot, a3);
-      }
-    } else if (mat == 2) {  // Glass :3577-3670
-      const float a = 1.5f - 1.0f, b = 1.5f + 1.0f, c = 1.0f / 1.5f, d = 1.5f / 1.0f;
-      const float e = (a * a) / (b * b);
-      const bool in = sX * wX + sY * wY + sZ * wZ > 0.0f;
-      const float nnt = in ? c : d;
-      const float dw = wX * dx + wY * dy + wZ * dz;
-      const float tir = 1.0f - nnt * nnt * (1.0f - dw * dw);
-      const float rX = dx - sX * dp2, rY = dy - sY * dp2, rZ = dz - sZ * dp2;
-      if (tir < 0.0f) {
-        ox = pX + wX * 0.01f;
-        oy = pY + wY * 0.01f;
-        oz = pZ + wZ * 0.01f;
-        dx = rX;
-        dy = rY;
-        dz = rZ;
-      } else {
-        const float st = jsqrt(tir);
-        const float sc = in ? dw * nnt + st : -(dw * nnt + st);
-        const float t0X = dx * nnt - sX * sc, t0Y = dy * nnt - sY * sc, t0Z = dz * nnt - sZ * sc;
-        const float tr = jrsqrt(t0X * t0X + t0Y * t0Y + t0Z * t0Z);
-        const float tX = t0X * tr, tY = t0Y * tr, tZ = t0Z * tr;
-        const float a1 = (in ? -dw : tX * sX + tY * sY + tZ * sZ);
-        const float a2 = 1.0f - a1;
-        const float refl = e + (1.0f - e) * a2 * a2 * a2 * a2 * a2;
-        const float trans = 1.0f - refl;
-        const float prob = 0.25f + 0.5f * refl;
-        const float rp = refl / prob;
-        const float tp = trans / (1.0f - prob);
-        const bool isR = next_float() < prob;
-        const float m = isR ? rp : tp;
-        mr *= m;
-        mg *= m;
-        mb *= m;
-        if (tan_) tan_->scale(m);
-        const float eps = isR ? 0.01f : 0.000001f;
-        ox = pX + wX * eps;
-        oy = pY + wY * eps;
-        oz = pZ + wZ * eps;
-        dx = isR ? rX : tX;
-        dy = isR ? rY : tY;
-        dz = isR ? rZ : tZ;
-      }
-    } else if (mat == 5) {  // Reflection :3671-3686
-      ox = pX + sX * 0.000001f;
-      oy = pY + sY * 0.000001f;
-      oz = pZ + sZ * 0.000001f;
-      dx = dx - sX * dp2;
-      dy = dy - sY * dp2;
-      dz = dz - sZ * dp2;
-      mr *= aR;
-      mg *= aG;
-      mb *= aB;
-      if (tan_) {
-        const float a3[3] = {aR, aG, aB};
-        tan_->albedo(aslot, a3);
-      }
-    }
+    scatter(mat, aslot, pX, pY, pZ, sX, sY, sZ, wX, wY, wZ, dp2, aR, aG, aB, ox, oy, oz, dx, dy, dz, mr, mg, mb);
   } while (depth < depth_max);
   const bool hit = do_intersect_primitives(ox, oy, oz, dx, dy, dz, true) < INFINITY_;
   if (!hit) {
@@ -1846,6 +1741,208 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
     pb += mb * tmp_[2] * PI_RECIPROCAL;
   }
   film_add(pr, pg, pb);
+}
+
+// The BSDF sample of material `mat` at a surface vertex (GPURendererKernel.java:3323-3686): replaces the ray
+// (origin o, direction d, on entry the incoming direction) with the next one and multiplies the throughput m,
+// given the hit point p, the shading normal s, the oriented normal w and dp2 = 2 s.d (orient()), and the
+// albedo a (divided by the Russian-roulette probability when that applies).  `aslot` is the albedo's
+// parameter group for the tangent (-1: none).  Draws 0 (Reflection, total internal reflection), 1 (Glass,
+// ClearCoat's reflection) or 2 (Lambertian, Phong) further random numbers, 3 for ClearCoat's diffuse lobe.
+void Item::scatter(int mat, int aslot, float pX, float pY, float pZ, float sX, float sY, float sZ, float wX,
+                   float wY, float wZ, float dp2, float aR, float aG, float aB, float& ox, float& oy, float& oz,
+                   float& dx, float& dy, float& dz, float& mr, float& mg, float& mb) {
+  if (mat == 1) {  // ClearCoat :3323-3464
+    const float a = 1.5f - 1.0f, b = 1.5f + 1.0f, c = 1.0f / 1.5f;
+    const float e = (a * a) / (b * b);
+    const float nnt = c;
+    const float dp0 = wX * dx + wY * dy + wZ * dz;
+    const float tir = 1.0f - nnt * nnt * (1.0f - dp0 * dp0);
+    const float rX = dx - sX * dp2, rY = dy - sY * dp2, rZ = dz - sZ * dp2;
+    if (tir < 0.0f) {
+      ox = pX + wX * 0.02f;
+      oy = pY + wY * 0.02f;
+      oz = pZ + wZ * 0.02f;
+      dx = rX;
+      dy = rY;
+      dz = rZ;
+    } else {
+      const float a1 = -dp0;
+      const float a2 = 1.0f - a1;
+      const float refl = e + (1.0f - e) * a2 * a2 * a2 * a2 * a2;
+      const float trans = 1.0f - refl;
+      const float prob = planted(PLANT_FRESNEL_PROB) ? refl : 0.25f + 0.5f * refl;
+      const float rp = refl / prob;
+      const float tp = trans / (1.0f - prob);
+      const bool isR = next_float() < prob;
+      const float m = isR ? rp : tp;
+      mr *= m;
+      mg *= m;
+      mb *= m;
+      if (tan_) tan_->scale(m);
+      if (isR) {
+        const float off = planted(PLANT_CLEARCOAT_OFFSET) ? 0.01f : 0.02f;
+        ox = pX + wX * off;
+        oy = pY + wY * off;
+        oz = pZ + wZ * off;
+        dx = rX;
+        dy = rY;
+        dz = rZ;
+      } else {
+        const float u = next_float();
+        const float v = next_float();
+        const float phi = PI_MULTIPLIED_BY_TWO * u;
+        const float ct = jsqrt(v), st = jsqrt(1.0f - v);
+        const float x = jcos(phi) * st, y = jsin(phi) * st, z = ct;
+        const bool isY = jabs(wX) > 0.1f;
+        const float uX = (isY ? 1.0f : 0.0f) * wZ;
+        const float uY = -((isY ? 0.0f : 1.0f) * wZ);
+        const float uZ = (isY ? 0.0f : 1.0f) * wY - (isY ? 1.0f : 0.0f) * wX;
+        const float ur = jrsqrt(uX * uX + uY * uY + uZ * uZ);
+        const float unX = uX * ur, unY = uY * ur, unZ = uZ * ur;
+        const float vnX = wY * unZ - wZ * unY, vnY = wZ * unX - wX * unZ, vnZ = wX * unY - wY * unX;
+        const float lX = unX * x + vnX * y + wX * z;
+        const float lY = unY * x + vnY * y + wY * z;
+        const float lZ = unZ * x + vnZ * y + wZ * z;
+        const float lr = jrsqrt(lX * lX + lY * lY + lZ * lZ);
+        ox = pX + wX * 0.01f;
+        oy = pY + wY * 0.01f;
+        oz = pZ + wZ * 0.01f;
+        dx = lX * lr;
+        dy = lY * lr;
+        dz = lZ * lr;
+        mr *= aR;
+        mg *= aG;
+        mb *= aB;
+        if (tan_) {
+          const float a3[3] = {aR, aG, aB};
+          tan_->albedo(aslot, a3);
+        }
+      }
+    }
+  } else if (mat == 3) {  // Lambertian :3465-3515
+    const float u = next_float();
+    const float v = next_float();
+    const float phi = PI_MULTIPLIED_BY_TWO * u;
+    const float ct = planted(PLANT_LAMBERT_UNIFORM) ? v : jsqrt(v);
+    const float st = planted(PLANT_LAMBERT_UNIFORM) ? jsqrt(1.0f - v * v) : jsqrt(1.0f - v);
+    const float x = jcos(phi) * st, y = jsin(phi) * st, z = ct;
+    const bool isY = jabs(wX) > 0.1f;
+    const float uX = (isY ? 1.0f : 0.0f) * wZ;
+    const float uY = -((isY ? 0.0f : 1.0f) * wZ);
+    const float uZ = (isY ? 0.0f : 1.0f) * wY - (isY ? 1.0f : 0.0f) * wX;
+    const float ur = jrsqrt(uX * uX + uY * uY + uZ * uZ);
+    const float unX = uX * ur, unY = uY * ur, unZ = uZ * ur;
+    const float vnX = wY * unZ - wZ * unY, vnY = wZ * unX - wX * unZ, vnZ = wX * unY - wY * unX;
+    const float lX = unX * x + vnX * y + wX * z;
+    const float lY = unY * x + vnY * y + wY * z;
+    const float lZ = unZ * x + vnZ * y + wZ * z;
+    const float lr = jrsqrt(lX * lX + lY * lY + lZ * lZ);
+    ox = pX + wX * 0.01f;
+    oy = pY + wY * 0.01f;
+    oz = pZ + wZ * 0.01f;
+    dx = lX * lr;
+    dy = lY * lr;
+    dz = lZ * lr;
+    mr *= aR;
+    mg *= aG;
+    mb *= aB;
+    if (tan_) {
+      const float a3[3] = {aR, aG, aB};
+      tan_->albedo(aslot, a3);
+    }
+  } else if (mat == 4) {  // Phong :3516-3576
+    const float exponent = planted(PLANT_PHONG_EXPONENT) ? 20.0f : 50.0f;
+    const float u = next_float();
+    const float v = next_float();
+    const float phi = PI_MULTIPLIED_BY_TWO * u;
+    const float ct = jpow(1.0f - v, 1.0f / (exponent + 1.0f));
+    const float st = jsqrt(jmaxf(0.0f, 1.0f - ct * ct));
+    const float x = jcos(phi) * st, y = jsin(phi) * st, z = ct;
+    const float rwX = dx - sX * dp2, rwY = dy - sY * dp2, rwZ = dz - sZ * dp2;
+    const float rwr = jrsqrt(rwX * rwX + rwY * rwY + rwZ * rwZ);
+    const float rnX = rwX * rwr, rnY = rwY * rwr, rnZ = rwZ * rwr;
+    const bool isY = jabs(rnX) > 0.1f;
+    const float uX = (isY ? 1.0f : 0.0f) * rnZ;
+    const float uY = -((isY ? 0.0f : 1.0f) * rnZ);
+    const float uZ = (isY ? 0.0f : 1.0f) * rnY - (isY ? 1.0f : 0.0f) * rnX;
+    const float ur = jrsqrt(uX * uX + uY * uY + uZ * uZ);
+    const float unX = uX * ur, unY = uY * ur, unZ = uZ * ur;
+    const float vnX = rnY * unZ - rnZ * unY, vnY = rnZ * unX - rnX * unZ, vnZ = rnX * unY - rnY * unX;
+    const float qX = unX * x + vnX * y + rwX * z;
+    const float qY = unY * x + vnY * y + rwY * z;
+    const float qZ = unZ * x + vnZ * y + rwZ * z;
+    const float qr = jrsqrt(qX * qX + qY * qY + qZ * qZ);
+    ox = pX + rnX * 0.01f;
+    oy = pY + rnY * 0.01f;
+    oz = pZ + rnZ * 0.01f;
+    dx = qX * qr;
+    dy = qY * qr;
+    dz = qZ * qr;
+    mr *= aR;
+    mg *= aG;
+    mb *= aB;
+    if (tan_) {
+      const float a3[3] = {aR, aG, aB};
+      tan_->albedo(aslot, a3);
+    }
+  } else if (mat == 2) {  // Glass :3577-3670
+    const float a = 1.5f - 1.0f, b = 1.5f + 1.0f, c = 1.0f / 1.5f, d = 1.5f / 1.0f;
+    const float e = (a * a) / (b * b);
+    const bool in = sX * wX + sY * wY + sZ * wZ > 0.0f;
+    const float nnt = planted(PLANT_SNELL_RATIO) ? (in ? d : c) : in ? c : d;
+    const float dw = wX * dx + wY * dy + wZ * dz;
+    const float tir = 1.0f - nnt * nnt * (1.0f - dw * dw);
+    const float rX = dx - sX * dp2, rY = dy - sY * dp2, rZ = dz - sZ * dp2;
+    if (tir < 0.0f) {
+      ox = pX + wX * 0.01f;
+      oy = pY + wY * 0.01f;
+      oz = pZ + wZ * 0.01f;
+      dx = rX;
+      dy = rY;
+      dz = rZ;
+    } else {
+      const float st = jsqrt(tir);
+      const float sc = in ? dw * nnt + st : -(dw * nnt + st);
+      const float t0X = dx * nnt - sX * sc, t0Y = dy * nnt - sY * sc, t0Z = dz * nnt - sZ * sc;
+      const float tr = jrsqrt(t0X * t0X + t0Y * t0Y + t0Z * t0Z);
+      const float tX = t0X * tr, tY = t0Y * tr, tZ = t0Z * tr;
+      const float a1 = (in ? -dw : tX * sX + tY * sY + tZ * sZ);
+      const float a2 = 1.0f - a1;
+      const float refl = e + (1.0f - e) * a2 * a2 * a2 * a2 * a2;
+      const float trans = 1.0f - refl;
+      const float prob = planted(PLANT_FRESNEL_PROB) ? refl : 0.25f + 0.5f * refl;
+      const float rp = refl / prob;
+      const float tp = trans / (1.0f - prob);
+      const bool isR = next_float() < prob;
+      const float m = isR ? rp : tp;
+      mr *= m;
+      mg *= m;
+      mb *= m;
+      if (tan_) tan_->scale(m);
+      const float eps = isR ? 0.01f : 0.000001f;
+      ox = pX + wX * eps;
+      oy = pY + wY * eps;
+      oz = pZ + wZ * eps;
+      dx = isR ? rX : tX;
+      dy = isR ? rY : tY;
+      dz = isR ? rZ : tZ;
+    }
+  } else if (mat == 5) {  // Reflection :3671-3686
+    ox = pX + sX * 0.000001f;
+    oy = pY + sY * 0.000001f;
+    oz = pZ + sZ * 0.000001f;
+    dx = dx - sX * dp2;
+    dy = dy - sY * dp2;
+    dz = dz - sZ * dp2;
+    mr *= aR;
+    mg *= aG;
+    mb *= aB;
+    if (tan_) {
+      const float a3[3] = {aR, aG, aB};
+      tan_->albedo(aslot, a3);
+    }
+  }
 }
 
 // The sky term m * sky(d) / pi of a path that left the scene: through the
@@ -2125,22 +2222,32 @@ void Item::image_pass() {  // run() :282-307 and AbstractImageKernel.java:410-47
 }
 
 void Item::run() {  // GPURendererKernel.run :252-308
-  const bool sky_active = s_.sky->is_sky_active == 1;
-  const bool sun_active = s_.sky->is_sun_active == 1;
   if (create_primary_ray()) {
-    switch (s_.p->renderer_type) {
-      case DF_RENDERER_AMBIENT_OCCLUSION: render_ao(); break;
-      case DF_RENDERER_PATH_TRACER: render_pt(sky_active, sun_active); break;
-      case DF_RENDERER_RAY_CASTER: render_rc(sky_active); break;
-      case DF_RENDERER_RAY_MARCHER: render_rm(sky_active); break;
-      case DF_RENDERER_RAY_TRACER: render_rt(sky_active); break;
-      case DF_RENDERER_SURFACE_NORMALS: render_sn(sky_active); break;
-      default: render_pt(sky_active, sun_active); break;
-    }
-    if (s_.p->wireframes == 1) render_wireframes();
+    shade_primary();
   } else {
     film_set(0.0f, 0.0f, 0.0f);
   }
+  image_pass();
+}
+
+void Item::shade_primary() {  // run() :258-300, the renderer dispatch and the wireframe overlay
+  const bool sky_active = s_.sky->is_sky_active == 1;
+  const bool sun_active = s_.sky->is_sun_active == 1;
+  switch (s_.p->renderer_type) {
+    case DF_RENDERER_AMBIENT_OCCLUSION: render_ao(); break;
+    case DF_RENDERER_PATH_TRACER: render_pt(sky_active, sun_active); break;
+    case DF_RENDERER_RAY_CASTER: render_rc(sky_active); break;
+    case DF_RENDERER_RAY_MARCHER: render_rm(sky_active); break;
+    case DF_RENDERER_RAY_TRACER: render_rt(sky_active); break;
+    case DF_RENDERER_SURFACE_NORMALS: render_sn(sky_active); break;
+    default: render_pt(sky_active, sun_active); break;
+  }
+  if (s_.p->wireframes == 1) render_wireframes();
+}
+
+void Item::run_ray(const float ray[6]) {
+  std::memcpy(ray_, ray, sizeof ray_);
+  shade_primary();
   image_pass();
 }
 
@@ -2338,6 +2445,88 @@ void oracle_sun_cosine_dual(const df_sky* frame, double turbidity, double elevat
   out[1] = c.e;
 }
 
+// doCreatePrimaryRay (GPURendererKernel.java:573-724) for pixel (x, y) of a width x height frame with the
+// camera array and renderer type given; advances *seed by the draws it makes.  Writes the ray (origin, unit
+// direction) and returns 1, or returns 0 where the lens gives no ray.
+int oracle_primary_ray(const float camera[19], int32_t renderer_type, int32_t width, int32_t height, int32_t x,
+                       int32_t y, int64_t* seed, float ray_out[6]) {
+  df_compiled_scene a{};
+  a.camera = camera;
+  a.camera_len = 19;
+  df_params p{};
+  p.renderer_type = renderer_type;
+  df_sky sk{};
+  Scene s{&a, &sk, &p, 0};
+  std::vector<int64_t> seeds(static_cast<size_t>(width) * height, 0);
+  const int gid = y * width + x;
+  seeds[gid] = *seed;
+  float film[3];
+  int32_t n = 0, po = -1;
+  uint8_t bgra[4];
+  Frame f{width, height, film, &n, nullptr, bgra, &po, seeds.data(), 0};
+  Counters c;
+  Item it(s, f, gid, c);
+  const bool ok = it.primary_ray(ray_out);
+  *seed = seeds[gid];
+  return ok ? 1 : 0;
+}
+
+// The path tracer's BSDF sample of material `material` (1 ClearCoat, 2 Glass, 3 Lambertian, 4 Phong,
+// 5 Reflection; GPURendererKernel.java:3323-3686) for a ray arriving along the unit direction d at the point p
+// with the shading normal s, the albedo given; advances *seed by the draws it makes.  out = next origin (3),
+// next direction (3), the throughput factor (3) the sample multiplies the path's throughput by.
+void oracle_scatter(int32_t material, const float p[3], const float s[3], const float d[3], const float albedo[3],
+                    int64_t* seed, float out[9]) {
+  df_compiled_scene a{};
+  df_params pr{};
+  df_sky sk{};
+  Scene sc{&a, &sk, &pr, 0};
+  float film[3];
+  int32_t n = 0, po = -1;
+  uint8_t bgra[4];
+  Frame f{1, 1, film, &n, nullptr, bgra, &po, seed, 0};
+  Counters c;
+  Item it(sc, f, 0, c);
+  float w[3], dp2;
+  Item::orient(s[0], s[1], s[2], d[0], d[1], d[2], w, &dp2);
+  float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = d[0], dy = d[1], dz = d[2], mr = 1.0f, mg = 1.0f, mb = 1.0f;
+  it.scatter(material, -1, p[0], p[1], p[2], s[0], s[1], s[2], w[0], w[1], w[2], dp2, albedo[0], albedo[1],
+             albedo[2], ox, oy, oz, dx, dy, dz, mr, mg, mb);
+  const float o9[9] = {ox, oy, oz, dx, dy, dz, mr, mg, mb};
+  std::memcpy(out, o9, sizeof o9);
+}
+
+// The sun direction doCalculateColorForSun samples from the point p with the random numbers (u, v)
+// (GPURendererKernel.java:2252-2320).
+void oracle_sun_direction(const df_sky* sky, const float p[3], float u, float v, float d_out[3]) {
+  df_compiled_scene a{};
+  df_params pr{};
+  Scene sc{&a, sky, &pr, 0};
+  int64_t seed = 0;
+  float film[3];
+  int32_t n = 0, po = -1;
+  uint8_t bgra[4];
+  Frame f{1, 1, film, &n, nullptr, bgra, &po, &seed, 0};
+  Counters c;
+  Item(sc, f, 0, c).sun_direction(p[0], p[1], p[2], u, v, d_out);
+}
+
+// run() for one pixel whose primary ray is given (origin, unit direction): the renderer of `params`, a cleared
+// film, the seed *seed (advanced).  Writes the film sample, the primitive offset buffer and the BGRA bytes.
+int oracle_render_ray(const df_compiled_scene* scene, const df_sky* sky, const df_params* params, const float ray[6],
+                      int64_t* seed, float rgb_out[3], int32_t* primitive_offset, uint8_t bgra_out[4]) {
+  if (!scene || !sky || !params || !ray || !seed || !rgb_out) return DF_EINVAL;
+  Scene s{scene, sky, params, static_cast<int>(scene->primitives_len / 3)};
+  int32_t n = 0, po = -1;
+  uint8_t bgra[4];
+  Frame f{1, 1, rgb_out, &n, nullptr, bgra, &po, seed, 1};
+  Counters c;
+  Item(s, f, 0, c).run_ray(ray);
+  if (primitive_offset) *primitive_offset = po;
+  if (bgra_out) std::memcpy(bgra_out, bgra, 4);
+  return 0;
+}
+
 // Single ray query against the scene (doIntersectPrimitives); writes the
 // 27-float hit record when testing == 0.  Returns t.
 float oracle_intersect(const df_compiled_scene* scene, const df_sky* sky, const df_params* params, const float o[3],
@@ -2450,6 +2639,14 @@ void oracle_tonemap(const df_params* params, const float rgb[3], int32_t primiti
 // Daylight chromaticity + luminance -> sRGB step of the sky (see above).
 void oracle_daylight_xyy_to_srgb(float x, float y, double luminance, float rgb_out[3]) {
   daylight_xyy_to_srgb(x, y, luminance, &rgb_out[0], &rgb_out[1], &rgb_out[2]);
+}
+
+// Switches on the deliberate mistake `id` (enum Plant; 0 restores the faithful restatement) for the pin tests
+// that show their checks fail on it.  Returns the previous value.
+int oracle_plant(int id) {
+  const int old = g_plant;
+  g_plant = id;
+  return old;
 }
 
 // The per-pixel RNG: advances *seed once and returns nextFloat().

@@ -235,3 +235,88 @@ def sun_cosine_dual(sky, turbidity, elevation, azimuth, point, normal, u, v):
     f(ctypes.addressof(sky), turbidity, elevation, azimuth, ctypes.addressof(p), ctypes.addressof(n), u, v,
       ctypes.addressof(out))
     return out[0], out[1]
+
+
+def primary_ray(camera, renderer_type, width, height, x, y, seed):
+    """doCreatePrimaryRay for pixel (x, y): (ray [origin, direction] or None where the lens gives none,
+    seed after the draws)."""
+    cam = np.ascontiguousarray(camera, np.float32)
+    s = ctypes.c_int64(seed)
+    out = (ctypes.c_float * 6)()
+    f = lib().oracle_primary_ray
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                  ctypes.c_void_p, ctypes.c_void_p]
+    ok = f(cam.ctypes.data, int(renderer_type), int(width), int(height), int(x), int(y), ctypes.addressof(s),
+           ctypes.addressof(out))
+    return (np.array(out[:], np.float32) if ok else None), s.value
+
+
+def scatter(material, point, normal, direction, albedo, seed):
+    """The path tracer's BSDF sample: (next origin, next direction, throughput factor, seed after the draws)."""
+    p = (ctypes.c_float * 3)(*point)
+    n = (ctypes.c_float * 3)(*normal)
+    d = (ctypes.c_float * 3)(*direction)
+    a = (ctypes.c_float * 3)(*albedo)
+    s = ctypes.c_int64(seed)
+    out = (ctypes.c_float * 9)()
+    f = lib().oracle_scatter
+    f.restype = None
+    f.argtypes = [ctypes.c_int32] + [ctypes.c_void_p] * 6
+    f(int(material), ctypes.addressof(p), ctypes.addressof(n), ctypes.addressof(d), ctypes.addressof(a),
+      ctypes.addressof(s), ctypes.addressof(out))
+    v = np.array(out[:], np.float32)
+    return v[0:3], v[3:6], v[6:9], s.value
+
+
+def sun_direction(sky, point, u, v):
+    """The direction doCalculateColorForSun samples from `point` with the random numbers (u, v)."""
+    p = (ctypes.c_float * 3)(*point)
+    out = (ctypes.c_float * 3)()
+    f = lib().oracle_sun_direction
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
+    f(ctypes.addressof(sky), ctypes.addressof(p), float(u), float(v), ctypes.addressof(out))
+    return np.array(out[:], np.float32)
+
+
+def render_ray(scene, params, ray, seed, sky=None):
+    """run() for one pixel whose primary ray is given: (film sample rgb, primitive offset, BGRA, seed after)."""
+    r = (ctypes.c_float * 6)(*ray)
+    s = ctypes.c_int64(seed)
+    rgb = (ctypes.c_float * 3)()
+    po = ctypes.c_int32(-1)
+    bgra = (ctypes.c_uint8 * 4)()
+    cs = scene.as_struct()
+    f = lib().oracle_render_ray
+    f.restype = ctypes.c_int
+    rc = f(ctypes.byref(cs), ctypes.byref(scene.sky if sky is None else sky), ctypes.byref(params), r,
+           ctypes.byref(s), rgb, ctypes.byref(po), bgra)
+    if rc != 0:
+        raise RuntimeError("oracle_render_ray failed: %d" % rc)
+    return np.array(rgb[:], np.float32), po.value, np.array(bgra[:], np.uint8), s.value
+
+
+# The deliberate mistakes of df_oracle.cpp's enum Plant, switched on one at a time by the pin tests.
+PLANTS = {name: i for i, name in enumerate([
+    "none", "camera_res", "camera_box", "lens_radius", "fisheye_w", "lambert_uniform", "phong_exponent",
+    "fresnel_prob", "snell_ratio", "sun_cone", "ao_uniform", "phong_shader", "sphere_uv", "barycentric",
+    "normal_map_b", "noise_normal", "bullseye", "perez_theta0", "perez_swap", "fbm_texture", "clearcoat_offset"])}
+
+
+class planted:
+    """Context manager: the oracle with the deliberate mistake `name` switched on (TEST INFRASTRUCTURE)."""
+
+    def __init__(self, name):
+        self.id = PLANTS[name]
+
+    def __enter__(self):
+        f = lib().oracle_plant
+        f.restype = ctypes.c_int
+        f.argtypes = [ctypes.c_int]
+        self.old = f(self.id)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_plant(self.old)
+        return False

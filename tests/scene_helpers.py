@@ -53,10 +53,10 @@ class SceneBuilder:
         return off
 
     def surface(self, material, albedo=(0.8, 0.8, 0.8), emission=(0.0, 0.0, 0.0), albedo_texture=None,
-                noise_amount=0.0, noise_scale=0.0):
+                noise_amount=0.0, noise_scale=0.0, normal_texture=None):
         a = self.constant_texture(albedo) if albedo_texture is None else albedo_texture
         e = self.constant_texture(emission)
-        n = self.constant_texture((0.0, 0.0, 0.0))
+        n = self.constant_texture((0.0, 0.0, 0.0)) if normal_texture is None else normal_texture
         off = len(self.surfaces)
         self.surfaces += [float(material), float(a), float(e), float(n), float(noise_amount), float(noise_scale)]
         return off
@@ -101,6 +101,20 @@ class SceneBuilder:
 
     def triangle(self, a, b, c, surface):
         return self._prim(4, self._triangle(a, b, c), surface)
+
+    def uv(self, uv):
+        self.p2 += [float(x) for x in uv]
+        return len(self.p2) - 2
+
+    def triangle_attrs(self, abc, normals, uvs, tangents, surface):
+        """A Triangle primitive with its own normal, texture coordinate and tangent at each vertex."""
+        pts = [self.point(np.array(p, np.float32)) for p in abc]
+        ns = [self.vector(n) for n in normals]
+        us = [self.uv(u) for u in uvs]
+        ts = [self.vector(t) for t in tangents]
+        off = len(self.triangles)
+        self.triangles += pts + ns + us + ts
+        return self._prim(4, off, surface)
 
     def mesh_single_leaf(self, tris, surface):
         """A TriangleMesh whose BVH is one leaf holding every triangle."""

@@ -1,0 +1,356 @@
+"""Pins of the oracle's sampling and lighting against what the mathematics fixes:
+
+* the path tracer's BSDF samples (GPURendererKernel.java:3323-3686): the Lambertian lobe is cosine-weighted
+  (cos^2 of the angle to the oriented normal uniform); Phong's lobe about the mirror direction has
+  cos^(n+1) uniform for n = 50; Reflection is the exact mirror; Glass and ClearCoat reflect with probability
+  0.25 + 0.5 R (Schlick's R, R0 = 0.04) and weight R / P or T / (1 - P); Glass refracts by Snell's law with
+  the index ratio of the side it enters from and reflects totally past the critical angle;
+* the sun sample (:2238-2347): uniform over the cone about the sun direction whose half-angle is that of a
+  disk of radius 500 at the sun's distance;
+* ambient occlusion (:3029-3149): an open plane is unoccluded; under a sphere the occluded fraction of a
+  cosine-weighted hemisphere is sin^2 of the sphere's angular radius;
+* the ray caster's Phong model (:1962-2039, :3707-3800): ambient 0.2 a + diffuse 0.5 a (n.l) + specular
+  0.5 (r.v)^250 toward a point light at the sun's origin, ambient only in shadow;
+* the path tracer's sun term (:3256-3272): albedo x cosine / pi at the sampled direction.
+
+Every check also runs against the oracle with one deliberate mistake switched on and must then fail."""
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import dayflower_amd as dfa
+from oracle import oracle_ffi
+from scene_helpers import SceneBuilder, default_camera
+
+P0 = np.array([0.3, -1.2, 2.0], np.float32)
+R0 = 0.04  # ((1.5 - 1) / (1.5 + 1))^2
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v)
+
+
+def _seeds(n, k):
+    return np.random.default_rng(k).integers(1, 1 << 48, n)
+
+
+def _lcg(seed):
+    return (seed * 0x5DEECE66D + 0xB) & ((1 << 48) - 1)
+
+
+def _frame(w):
+    t = _unit(np.cross(w, [0.3, 0.5, 0.8]))
+    return t, np.cross(w, t)
+
+
+def _schlick(c):
+    return R0 + (1 - R0) * (1 - c) ** 5
+
+
+def _incoming(theta_deg, s, from_outside=True):
+    """A unit direction arriving at angle theta to the normal s, from the side s points to or the other."""
+    t, _ = _frame(s)
+    th = np.radians(theta_deg)
+    d = np.sin(th) * t - np.cos(th) * s
+    return (d if from_outside else d + 2 * np.cos(th) * s).astype(np.float32)
+
+
+def _sample(mat, s, d, albedo, n, k):
+    out = []
+    for seed in _seeds(n, k):
+        o, d2, m, after = oracle_ffi.scatter(mat, P0, s, d, albedo, int(seed))
+        out.append((o.astype(np.float64), d2.astype(np.float64), m.astype(np.float64), int(seed), after))
+    return out
+
+
+def _uniform_p(v):
+    return stats.kstest(v, "uniform").pvalue
+
+
+def _lambert_ok(n=6000):
+    s = _unit([0.2, 0.9, -0.3]).astype(np.float32)
+    a = (0.7, 0.5, 0.25)
+    for outside in (True, False):
+        d = _incoming(35.0, s.astype(np.float64), outside)
+        w = s if outside else -s
+        t1, t2 = _frame(w.astype(np.float64))
+        c2, phi = [], []
+        for o, d2, m, seed, after in _sample(3, s, d, a, n, 3):
+            if after != _lcg(_lcg(seed)) or not np.allclose(m, a, rtol=1e-7):
+                return False
+            if np.abs(o - (P0 + 0.01 * w)).max() > 1e-6:
+                return False
+            c = d2 @ w
+            if c < -1e-6:
+                return False
+            c2.append(min(c * c, 1.0))
+            phi.append((np.arctan2(d2 @ t2, d2 @ t1) + np.pi) / (2 * np.pi))
+        if _uniform_p(c2) < 1e-3 or _uniform_p(phi) < 1e-3:
+            return False
+    return True
+
+
+def test_lambertian_lobe_is_cosine_weighted():
+    assert _lambert_ok()
+    with oracle_ffi.planted("lambert_uniform"):
+        assert not _lambert_ok(3000)
+
+
+def _phong_ok(n=6000):
+    s = _unit([0.0, 1.0, 0.2]).astype(np.float32)
+    d = _incoming(50.0, s.astype(np.float64))
+    a = (0.9, 0.8, 0.1)
+    r = _unit(d - 2 * (s @ d) * s)
+    c51 = []
+    for o, d2, m, seed, after in _sample(4, s, d, a, n, 4):
+        if after != _lcg(_lcg(seed)) or not np.allclose(m, a, rtol=1e-7):
+            return False
+        if np.abs(o - (P0 + 0.01 * r)).max() > 1e-6:
+            return False
+        c51.append(max(d2 @ r, 0.0) ** 51)
+    return _uniform_p(c51) > 1e-3
+
+
+def test_phong_lobe_about_the_mirror_direction():
+    assert _phong_ok()
+    with oracle_ffi.planted("phong_exponent"):
+        assert not _phong_ok(3000)
+
+
+def test_reflection_is_the_exact_mirror():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        s = _unit(rng.normal(size=3)).astype(np.float32)
+        d = _unit(rng.normal(size=3)).astype(np.float32)
+        seed = int(rng.integers(1, 1 << 48))
+        o, d2, m, after = oracle_ffi.scatter(5, P0, s, d, (0.5, 0.6, 0.7), seed)
+        sd, dd = s.astype(np.float64), d.astype(np.float64)
+        np.testing.assert_allclose(d2, dd - 2 * (sd @ dd) * sd, atol=2e-7)
+        np.testing.assert_allclose(o, P0 + 1e-6 * sd, atol=1e-6)
+        np.testing.assert_allclose(m, (0.5, 0.6, 0.7), rtol=1e-7)
+        assert after == seed  # no random numbers
+
+
+def _fresnel_case(mat, theta, outside, n, k):
+    """(reflect fraction, reflections, transmissions, expected R) of material `mat` at incidence theta."""
+    s = np.array([0.0, 1.0, 0.0], np.float32)
+    d = _incoming(theta, s.astype(np.float64), outside)
+    w = s if outside else -s
+    refl, trans = [], []
+    for o, d2, m, seed, after in _sample(mat, s, d, (0.6, 0.6, 0.6), n, 6 + mat):
+        (refl if d2 @ w > 0 and np.allclose(d2, d - 2 * (s @ d) * s, atol=2e-6) else trans).append((o, d2, m))
+    return s.astype(np.float64), d.astype(np.float64), w.astype(np.float64), refl, trans
+
+
+def _glass_ok(n=6000):
+    for theta, outside in ((60.0, True), (20.0, True), (30.0, False)):
+        s, d, w, refl, trans = _fresnel_case(2, theta, outside, n, 1)
+        ratio = 1 / 1.5 if outside else 1.5
+        sin_t = np.sin(np.radians(theta)) * ratio
+        cos_t = np.sqrt(1 - sin_t ** 2)
+        # entering: Schlick at the incident angle; leaving: at the transmitted one (:3633)
+        R = _schlick(np.cos(np.radians(theta)) if outside else cos_t)
+        P = 0.25 + 0.5 * R
+        f = len(refl) / n
+        if abs(f - P) > 5 * np.sqrt(P * (1 - P) / n):
+            return False
+        for o, d2, m in refl:
+            if not np.allclose(m, R / P, rtol=2e-5) or np.abs(o - (P0 + 0.01 * w)).max() > 1e-6:
+                return False
+        for o, d2, m in trans:
+            if not np.allclose(m, (1 - R) / (1 - P), rtol=2e-5) or np.abs(o - (P0 + 1e-6 * w)).max() > 1e-6:
+                return False
+            # Snell: the sine to the normal scales by the index ratio, in the plane of incidence, across
+            if abs(np.linalg.norm(np.cross(d2, s)) - sin_t) > 2e-6 or abs(d2 @ np.cross(d, s)) > 2e-6:
+                return False
+            if d2 @ w > 0:
+                return False
+    # from inside past the critical angle asin(1/1.5) = 41.8 degrees: the mirror, no random number
+    s, d, w, refl, trans = _fresnel_case(2, 50.0, False, 50, 2)
+    return len(trans) == 0 and all(np.abs(o - (P0 + 0.01 * w)).max() < 1e-6 and np.allclose(m, 1) for o, _, m in refl)
+
+
+def test_glass_fresnel_split_and_snell():
+    assert _glass_ok()
+    for mistake in ("fresnel_prob", "snell_ratio"):
+        with oracle_ffi.planted(mistake):
+            assert not _glass_ok(3000), mistake
+
+
+def _clearcoat_ok(n=6000):
+    s, d, w, refl, trans = _fresnel_case(1, 60.0, True, n, 3)
+    R = _schlick(np.cos(np.radians(60.0)))
+    P = 0.25 + 0.5 * R
+    if abs(len(refl) / n - P) > 5 * np.sqrt(P * (1 - P) / n):
+        return False
+    for o, d2, m in refl:
+        if not np.allclose(m, R / P, rtol=2e-5) or np.abs(o - (P0 + 0.02 * w)).max() > 1e-6:
+            return False
+    c2 = []
+    for o, d2, m in trans:  # the diffuse base: cosine-weighted, weighted T / (1 - P) times the albedo
+        if not np.allclose(m, 0.6 * (1 - R) / (1 - P), rtol=2e-5) or np.abs(o - (P0 + 0.01 * w)).max() > 1e-6:
+            return False
+        c2.append(min((d2 @ w) ** 2, 1.0))
+    return _uniform_p(c2) > 1e-3
+
+
+def test_clearcoat_coat_and_diffuse_base():
+    assert _clearcoat_ok()
+    for mistake in ("fresnel_prob", "clearcoat_offset"):
+        with oracle_ffi.planted(mistake):
+            assert not _clearcoat_ok(3000), mistake
+
+
+# ---------------------------------------------------------------- sun
+
+def _sun_ok():
+    sky = dfa.sky_at(30.0, 40.0, 3.0)
+    axis = np.array(sky.sun_direction_world[:], np.float64)
+    p = np.array([1.0, 0.5, -2.0], np.float32)
+    dist = np.linalg.norm(np.array(sky.sun_origin[:], np.float64) - p)
+    one_minus_cmax = 1 - np.sqrt(1 - (500.0 / dist) ** 2)
+    rng = np.random.default_rng(8)
+    for u, v in rng.uniform(0, 1, (300, 2)):
+        d = oracle_ffi.sun_direction(sky, p, u, v).astype(np.float64)
+        # cos of the angle to the axis is linear in u from 1 to cos(half-angle): uniform in solid angle
+        if abs((1 - d @ axis) - u * one_minus_cmax) > 3e-7 + 1e-3 * one_minus_cmax:
+            return False
+    # and the azimuth about the axis advances by 2 pi v
+    t1, t2 = _frame(axis)
+    prev = None
+    for v in np.arange(0.0, 1.0, 0.1):
+        d = oracle_ffi.sun_direction(sky, p, 0.7, v).astype(np.float64)
+        ang = np.arctan2(d @ t2, d @ t1)
+        if prev is not None:
+            step = (ang - prev) % (2 * np.pi)
+            if min(abs(step - 0.2 * np.pi), abs(step - 1.8 * np.pi)) > 2e-3:
+                return False
+        prev = ang
+    return True
+
+
+def test_sun_samples_the_cone_of_its_disk():
+    assert _sun_ok()
+    with oracle_ffi.planted("sun_cone"):
+        assert not _sun_ok()
+
+
+def _plane_scene(sphere=None, albedo=(0.8, 0.8, 0.8), sky=None, material=3):
+    b = SceneBuilder()
+    s = b.surface(material, albedo=albedo)
+    b.plane((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), s)
+    if sphere is not None:
+        b.sphere(sphere[0], sphere[1], b.surface(3))
+    return b.build(default_camera(), sky=sky)
+
+
+def test_path_tracer_sun_term_is_albedo_cosine_over_pi():
+    """One path-traced vertex on a Lambertian plane with the sky off (0.01 grey): the sample is
+    a (d_sun . n) / pi for the sun direction drawn with the pixel's third and fourth random numbers, plus the
+    escaped bounce's a 0.01 / pi; an occluder between the point and the sun leaves only the latter (or nothing,
+    where the bounce hits the occluder too)."""
+    sky = dfa.sky_at(50.0, 30.0, 2.0)
+    sky.is_sky_active = 0
+    a = np.float32(204 / 255)  # albedo 0.8 as an 8-bit texture
+    for occluded in (False, True):
+        l = np.array(sky.sun_direction_world[:], np.float64)
+        sc = _plane_scene(sphere=(tuple(l * 3.0), 0.5) if occluded else None, sky=sky)
+        p = sc.copy_params()
+        p.renderer_type = 2
+        p.pt_depth_maximum = 1
+        escaped = 0
+        for seed in _seeds(50, 11):
+            ray = np.array([0.0, 2.0, 0.0, 0.0, -1.0, 0.0], np.float32)
+            rgb, po, _, after = oracle_ffi.render_ray(sc, p, ray, int(seed))
+            s1 = _lcg(int(seed))
+            s2 = _lcg(s1)
+            u, v = (s1 >> 24) / float(1 << 24), (s2 >> 24) / float(1 << 24)
+            dsun = oracle_ffi.sun_direction(sky, (0.0, 0.0, 0.0), np.float32(u), np.float32(v)).astype(np.float64)
+            sun = 0.0 if occluded else a * max(dsun[1], 0.0) / np.pi
+            if occluded and np.all(rgb == 0.0):
+                continue
+            np.testing.assert_allclose(rgb, sun + a * 0.01 / np.pi, rtol=2e-6)
+            assert after == _lcg(_lcg(_lcg(_lcg(int(seed)))))  # sun u, v and the Lambertian u, v
+            escaped += 1
+        assert escaped >= 40
+
+
+# ---------------------------------------------------------------- ambient occlusion
+
+def _ao_mean(scene, n, k, max_distance=200.0):
+    p = scene.copy_params()
+    p.renderer_type = 1
+    p.ao_maximum_distance = max_distance
+    ray = np.concatenate([[3.0, 1.0, 0.0], _unit([-3.0, -1.0, 0.0])]).astype(np.float32)
+    vals = []
+    for seed in _seeds(n, k):
+        rgb, po, _, _ = oracle_ffi.render_ray(scene, p, ray, int(seed))
+        assert po == 0 and rgb[0] == rgb[1] == rgb[2]
+        vals.append(rgb[0])
+    return np.array(vals)
+
+
+def _ao_sphere_ok(n=4000):
+    r, h = 1.0, 2.0
+    vals = _ao_mean(_plane_scene(sphere=((0.0, h, 0.0), r)), n, 12)
+    occluded = (r / (h - 0.01)) ** 2  # the AO ray leaves 0.01 above the plane
+    mean = 1 - occluded
+    return abs(vals.mean() - mean) < 5 * np.sqrt(occluded * (1 - occluded) / n)
+
+
+def test_ambient_occlusion_closed_forms():
+    # an open plane is never occluded; a sphere beyond the maximum distance does not count
+    assert np.all(_ao_mean(_plane_scene(), 500, 13) == 1.0)
+    assert np.all(_ao_mean(_plane_scene(sphere=((0.0, 2.0, 0.0), 1.0)), 500, 14, max_distance=0.5) == 1.0)
+    assert _ao_sphere_ok()
+    with oracle_ffi.planted("ao_uniform"):
+        assert not _ao_sphere_ok()
+
+
+# ---------------------------------------------------------------- ray caster
+
+def _phong_bytes(a, n, l, v, shadowed):
+    """The ray caster's colour in float64: ambient + diffuse + specular toward a point light, 8-bit packed."""
+    r = 2 * (l @ n) * n - l
+    out = []
+    for ch in a:
+        c = 0.2 * ch
+        if not shadowed:
+            c += 0.5 * ch * max(l @ n, 0.0)
+            if l @ n > 0:
+                c += 0.5 * max(_unit(r) @ v, 0.0) ** 250
+        out.append(int(min(max(c, 0.0), 1.0) * 255 + 0.5))
+    return np.array(out)
+
+
+def _ray_caster_ok():
+    sky = dfa.sky_at(40.0, 70.0, 2.0)
+    origin = np.array(sky.sun_origin[:], np.float64)
+    a = (153 / 255, 102 / 255, 51 / 255)  # (0.6, 0.4, 0.2) as 8-bit
+    nrm = np.array([0.0, 1.0, 0.0])
+    for shadowed in (False, True):
+        l = _unit(origin)
+        sc = _plane_scene(albedo=(0.6, 0.4, 0.2), sky=sky, sphere=(tuple(l * 3.0), 0.5) if shadowed else None)
+        p = sc.copy_params()
+        p.renderer_type = 3
+        mirror = 2 * (l @ nrm) * nrm - l
+        t1, _ = _frame(mirror)
+        for off in (0.0, 0.01, 0.03, 0.05, 0.08, 0.3):
+            v = _unit(mirror + off * t1)  # towards the eye
+            ray = np.concatenate([5.0 * v, -v]).astype(np.float32)
+            rgb, po, _, _ = oracle_ffi.render_ray(sc, p, ray, 1)
+            hit = ray[:3].astype(np.float64) + ray[3:].astype(np.float64) * (5.0 * v[1] / -ray[4])
+            lp = _unit(origin - hit)
+            want = _phong_bytes(a, nrm, lp, v, shadowed)
+            got = np.round(rgb.astype(np.float64) * 255).astype(int)
+            if po != 0 or np.abs(got - want).max() > 1:
+                return False
+    return True
+
+
+def test_ray_caster_phong_closed_form():
+    assert _ray_caster_ok()
+    with oracle_ffi.planted("phong_shader"):
+        assert not _ray_caster_ok()
