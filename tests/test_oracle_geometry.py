@@ -198,3 +198,29 @@ def test_terrain_march_against_dense_fp64_march():
         assert t32 == pytest.approx(t_true, rel=0.011, abs=2e-3)
         checked += 1
     assert checked > 20
+
+
+def test_terrain_normal_is_the_central_difference_gradient():
+    """The terrain's normal (GPURendererKernel.java:2495-2539) is (h(x-e) - h(x+e), 2e, h(z-e) - h(z+e))
+    normalised, e = 0.02, at the marched point: within O(e^2) of the analytic normal
+    (-cos x sin z, 1, -sin x cos z) / |.| of h = sin x sin z."""
+    b = SceneBuilder()
+    b.terrain(b.surface(3))
+    sc = b.build(default_camera())
+    rng = np.random.default_rng(12)
+    checked = 0
+    for _ in range(60):
+        o = np.array([rng.uniform(-5, 5), rng.uniform(2, 6), rng.uniform(-5, 5)], np.float32)
+        d = np.array([rng.uniform(-1, 1), rng.uniform(-1, -0.2), rng.uniform(-1, 1)])
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        t, rec = oracle_ffi.intersect(sc, _params(), o, d)
+        if t >= INF:
+            continue
+        x, z = float(rec[2]), float(rec[4])
+        n = np.array([-np.cos(x) * np.sin(z), 1.0, -np.sin(x) * np.cos(z)])
+        n /= np.linalg.norm(n)
+        np.testing.assert_allclose(rec[19:22], n, atol=2e-4)
+        np.testing.assert_allclose(rec[5:8], rec[19:22], atol=0)
+        assert rec[8] == rec[2] and rec[9] == rec[4]  # uv = (x, z) of the marched point
+        checked += 1
+    assert checked > 20

@@ -354,3 +354,96 @@ def test_ray_caster_phong_closed_form():
     assert _ray_caster_ok()
     with oracle_ffi.planted("phong_shader"):
         assert not _ray_caster_ok()
+
+
+# ---------------------------------------------------------------- path tracer composition
+
+def _emitter_scene(albedo, emission):
+    sky = dfa.sky_at(50.0, 30.0, 2.0)
+    sky.is_sky_active = 0
+    sky.is_sun_active = 0
+    b = SceneBuilder()
+    b.plane((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), b.surface(3, albedo=albedo, emission=emission))
+    return b.build(default_camera(8, 8, eye=(0.0, 3.0, 0.5), target=(0.0, 0.0, 0.0)), sky=sky)
+
+
+def _q(c):
+    return np.float32(int(np.float32(c) * 255 + 0.5)) * np.float32(1 / 255)
+
+
+ALBEDO, EMISSION = (0.6, 0.3, 0.9), (0.2, 0.1, 0.05)
+
+
+def test_path_tracer_accumulates_emission_and_escape():
+    """Sky and sun off (the sky a constant 0.01 grey): a ray that misses scores 0.01 / pi; one that hits the
+    emissive Lambertian plane scores its emission, then its bounce escapes and adds albedo x 0.01 / pi,
+    L = Le + beta Lsky with beta the albedo (:3198-3211, :3274-3281, :3689-3701)."""
+    sc = _emitter_scene(ALBEDO, EMISSION)
+    p = sc.copy_params()
+    p.renderer_type = 2
+    p.pt_depth_russian_roulette = 10
+    a = np.array([_q(c) for c in ALBEDO], np.float64)
+    e = np.array([_q(c) for c in EMISSION], np.float64)
+    miss = np.array([0.0, 1.0, 0.0, 0.0, 1.0, 0.0], np.float32)
+    rgb, po, _, _ = oracle_ffi.render_ray(sc, p, miss, 5)
+    assert po == -1
+    np.testing.assert_allclose(rgb, 0.01 / np.pi, rtol=1e-6)
+    down = np.array([0.2, 2.0, 0.1, 0.0, -1.0, 0.0], np.float32)
+    for seed in _seeds(100, 15):
+        rgb, po, _, _ = oracle_ffi.render_ray(sc, p, down, int(seed))
+        assert po == 0
+        np.testing.assert_allclose(rgb, e + a * 0.01 / np.pi, rtol=1e-6)
+
+
+def test_russian_roulette_is_unbiased():
+    """Russian roulette from depth 1 (:3283-3321): the path continues with probability max(albedo) and its
+    throughput is divided by it; a terminated path re-tests its ray, which hits, so it keeps only what it
+    scored.  Frequency, both outcomes and the expectation match."""
+    sc = _emitter_scene(ALBEDO, EMISSION)
+    p = sc.copy_params()
+    p.renderer_type = 2
+    p.pt_depth_russian_roulette = 1
+    a = np.array([_q(c) for c in ALBEDO], np.float64)
+    e = np.array([_q(c) for c in EMISSION], np.float64)
+    pdf = a.max()
+    down = np.array([0.2, 2.0, 0.1, 0.0, -1.0, 0.0], np.float32)
+    n = 4000
+    cont = 0
+    total = np.zeros(3)
+    for seed in _seeds(n, 16):
+        rgb, _, _, _ = oracle_ffi.render_ray(sc, p, down, int(seed))
+        rgb = rgb.astype(np.float64)
+        if np.allclose(rgb, e, rtol=1e-6):
+            pass
+        else:
+            np.testing.assert_allclose(rgb, e + a / pdf * 0.01 / np.pi, rtol=1e-5)
+            cont += 1
+        total += rgb
+    assert abs(cont / n - pdf) < 5 * np.sqrt(pdf * (1 - pdf) / n)
+    mean = total / n
+    np.testing.assert_allclose(mean, e + a * 0.01 / np.pi, rtol=5 * np.sqrt((1 - pdf) / pdf / n) * 0.01 / np.pi / e.min() + 1e-6)
+
+
+def test_film_is_the_running_mean_and_clear_overwrites():
+    """filmAddColor (AbstractImageKernel.java:333-360): after n passes the film is the mean of the n samples
+    (each sample taken alone by a cleared single pass from the same seeds), the count is n, and a cleared pass
+    overwrites (filmSetColor semantics of the clear flag)."""
+    sc = _emitter_scene(ALBEDO, EMISSION)
+    p = sc.copy_params()
+    p.pt_depth_russian_roulette = 1
+    w = h = 8
+    seeds = dfa.pixel_seeds(17, w * h)
+    run = oracle_ffi.OracleState(w, h, seeds)
+    one = oracle_ffi.OracleState(w, h, seeds)
+    samples = []
+    for k in range(40):
+        oracle_ffi.render(sc, p, one, 1, clear_first=True)
+        samples.append(one.film.astype(np.float64).copy())
+    oracle_ffi.render(sc, p, run, 40)
+    np.testing.assert_array_equal(run.seeds, one.seeds)
+    np.testing.assert_allclose(run.film, np.mean(samples, axis=0), rtol=2e-6, atol=1e-9)
+    assert np.all(run.film_samples == 40)
+    oracle_ffi.render(sc, p, run, 1, clear_first=True)
+    oracle_ffi.render(sc, p, one, 1, clear_first=True)
+    np.testing.assert_array_equal(run.film, one.film)
+    assert np.all(run.film_samples == 1)
