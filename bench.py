@@ -13,7 +13,11 @@ Besides the contract fields the JSON line carries:
                  measured live (device counters; HIP events around every
                  launch on the renderer's stream), against the HBM peak when
                  its FLOP per byte is below the FP32 / HBM balance point,
-                 else against the FP32 peak;
+                 else against the FP32 peak; `measured` sets the same rates
+                 against this GPU's measured roofs (df_measure_roof: L2,
+                 MALL and HBM read bandwidth, unpacked FP32 FMA rate) and,
+                 from the committed rocprof issue counters, the kernel's
+                 VALU issue rate and active-lane fraction;
   stages       - kernel time, launches and ray queries per step of each stage;
   batch        - the batch mix (256 scenes of 64^2..512^2 pixels and 8..128 samples per
                  pixel): Mrays/s rendering the scenes one after another (plain)
@@ -23,7 +27,12 @@ Besides the contract fields the JSON line carries:
                  adjoint, and their ratio;
   cpu_baseline - the CPU oracle (the reference's per-pixel kernel restated in
                  C++ with the reference's thread-pool semantics) on a band of
-                 rows of the same frame, on this host's cores.
+                 rows of the same frame, on this host's CPU share (at most 16
+                 threads; the host's logical CPU count is reported beside).
+
+`traced_mrays_s` in `config` counts only the queries that are traced: the
+Russian-roulette exit re-tests its just-hit ray with the any-hit query, which
+always hits, so the device counts it as a ray without tracing it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -58,6 +67,57 @@ KERNEL_NAMES = {"megakernel": "render_kernel (one work-item per pixel)", "raygen
                 "trace": "wf_trace<0,false> (a bounce's closest-hit + sun any-hit queries)", "shade": "wf_shade",
                 "escape": "wf_trace<0,true> (escape tests + the last sun queries)", "film": "wf_film"}
 PEAK_HBM_GBS = 8000.0
+
+
+def measured_roofs():
+    """This GPU's sustained rates (df_measure_roof): GB/s read from L2, MALL and HBM, TFLOP/s of unpacked FP32
+    FMA, and the VALU wave-instruction issue rate that implies (one v_fma_f32 = 64 lanes x 2 FLOP)."""
+    import dayflower_amd as dfa
+
+    out = {}
+    for kind, key, scale in (("l2_read", "l2_gbs", 1e-9), ("mall_read", "mall_gbs", 1e-9),
+                             ("hbm_read", "hbm_gbs", 1e-9), ("valu_fp32", "valu_fp32_tflops", 1e-12)):
+        rate, _ = dfa.measure_roof(kind)
+        out[key] = round(rate * scale, 2)
+    out["valu_winst_per_s"] = out["valu_fp32_tflops"] * 1e12 / 128.0
+    return out
+
+
+def issue_from_profiles(workload, kernel_symbol):
+    """The kernel's VALU issue counters per dispatch from profiles/issue_workload<N>.json, or None."""
+    path = os.path.join(REPO, "profiles", "issue_workload%d.json" % workload)
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get("kernels", {}).get(kernel_symbol)
+
+
+def issue_summary(out_path, csv_paths, workload):
+    """VALU issue counters per dispatch of each renderer kernel from rocprofv3 --pmc counter_collection.csv
+    files (SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, SQ_THREAD_CYCLES_VALU, SQ_WAVES and the like): their means per
+    dispatch and the active-lane fraction SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)."""
+    import csv
+
+    per = {}
+    for path in csv_paths:
+        for r in csv.DictReader(open(path)):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            if not name.startswith(("dfk::", "dfk_roof::")):
+                continue
+            per.setdefault(name, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    kernels = {}
+    for name, counters in per.items():
+        k = {c: round(sum(v) / len(v), 1) for c, v in counters.items()}
+        k["dispatches"] = max(len(v) for v in counters.values())
+        if k.get("SQ_ACTIVE_INST_VALU"):
+            if "SQ_THREAD_CYCLES_VALU" in k:
+                k["active_lane_fraction"] = round(k["SQ_THREAD_CYCLES_VALU"] / (64.0 * k["SQ_ACTIVE_INST_VALU"]), 4)
+        kernels[name] = k
+    summary = {"workload": workload, "files": [os.path.basename(os.path.dirname(p)) for p in csv_paths],
+               "kernels": kernels}
+    with open(out_path, "w") as fh:
+        json.dump(summary, fh, indent=1)
+        fh.write("\n")
+    print(json.dumps(summary))
 
 
 def log(*a):
@@ -189,7 +249,10 @@ def adjoint_measure():
 
 def batch_measure(n_scenes):
     """The uneven batch mix (scene seeds 100.., each at its own resolution and samples per pixel): every scene
-    rendered alone one after another (plain, the best single-scene schedule), then all in one batched launch."""
+    rendered alone one after another (plain, the best single-scene schedule), then all in one batched launch.
+    Before the plain leg's clock starts, every scene has settled its schedule (the autotuner's timing passes,
+    then the chosen schedule fixed) and allocated its path state, and its seeds and film are reset, so the
+    timed region holds only the renders."""
     import dayflower_amd as dfa
 
     scenes = [dfa.build_batch_scene(100 + i) for i in range(n_scenes)]
@@ -198,6 +261,19 @@ def batch_measure(n_scenes):
         for i, sc in enumerate(scenes):
             b.add(sc, passes=sc.spp, seed=1 + i)
         jobs = [b.job(i) for i in range(n_scenes)]
+        engines = {dfa.PT_MEGAKERNEL: 0, dfa.PT_WAVEFRONT: 0}
+        for i, j in enumerate(jobs):
+            while j.engine() == dfa.PT_AUTO:
+                j.execute(1)
+            e = j.engine()
+            engines[e] += 1
+            p = j.getParams()
+            p.pt_engine = e
+            j.setParams(p)
+            j.execute(1)  # the fixed schedule's first pass (allocations), untimed
+            j.seed(1 + i)
+            j.clear()
+            j.synchronize()
         for j in jobs:
             j.resetCounters()
         t0 = time.perf_counter()
@@ -219,7 +295,8 @@ def batch_measure(n_scenes):
     return {"scenes": n_scenes, "samples": int(sum(sc.width * sc.height * sc.spp for sc in scenes)),
             "plain_mrays_s": round(plain, 2), "plain_s": round(t1 - t0, 3), "batched_mrays_s": round(batched, 2),
             "batched_s": round(t3 - t2, 3), "batched_over_plain": round(batched / plain, 3),
-            "rays_plain": rays_plain, "rays_batched": rays_batched}
+            "rays_plain": rays_plain, "rays_batched": rays_batched,
+            "plain_schedules": {"megakernel": engines[dfa.PT_MEGAKERNEL], "wavefront": engines[dfa.PT_WAVEFRONT]}}
 
 
 def main():
@@ -237,6 +314,11 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB float copy kernel (negation), the reference for scaling the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--calibrate-roofs", action="store_true",
+                    help="run only the roof microbenchmarks (df_measure_roof) and print their rates")
+    ap.add_argument("--issue-summary", nargs="+", metavar="PATH",
+                    help="OUT CSV...: write the VALU issue summary OUT from rocprofv3 --pmc counter_collection.csv "
+                         "files of this bench (SQ_* counters)")
     ap.add_argument("--traffic-summary", nargs=5, metavar=("OUT", "FETCH_CSV", "WRITE_CSV", "CAL_FETCH_CSV",
                                                            "CAL_WRITE_CSV"),
                     help="write the HBM-traffic summary OUT from rocprofv3 --pmc counter_collection.csv files of "
@@ -244,6 +326,9 @@ def main():
     args = ap.parse_args()
     if args.traffic_summary:
         traffic_summary(*args.traffic_summary, workload=args.workload)
+        return
+    if args.issue_summary:
+        issue_summary(args.issue_summary[0], args.issue_summary[1:], workload=args.workload)
         return
 
     if args.gpus != 1:
@@ -267,6 +352,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "torch.neg float32", "bytes_read_per_launch": n * 4,
                           "bytes_written_per_launch": n * 4, "launches": 3}), flush=True)
+        return
+    if args.calibrate_roofs:
+        print(json.dumps(measured_roofs()), flush=True)
         return
 
     scene = dfa.build_config(args.workload)
@@ -326,6 +414,20 @@ def main():
         bound, achieved, peak, unit = "mfma", achieved_tflops, PEAK_FP32_TFLOPS, "TFLOP/s"
         note = ("%.3g FLOP per algorithmic byte, above the FP32 / HBM balance of %.3g: the FP32 peak bounds it "
                 "(no matrix work)" % (intensity, balance))
+    roofs = measured_roofs()
+    issue = issue_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, ""))
+    measured = {
+        "roofs": roofs,
+        "alg_bytes_over_l2": round(achieved_gbs / roofs["l2_gbs"], 4),
+        "alg_bytes_over_mall": round(achieved_gbs / roofs["mall_gbs"], 4),
+        "alg_flops_over_valu": round(achieved_tflops / roofs["valu_fp32_tflops"], 4),
+    }
+    if issue and issue.get("SQ_INSTS_VALU"):
+        measured["valu_insts_per_launch"] = issue["SQ_INSTS_VALU"]
+        measured["valu_issue_over_peak"] = round(issue["SQ_INSTS_VALU"] / avg_launch_s / roofs["valu_winst_per_s"], 4)
+        if "active_lane_fraction" in issue:
+            measured["active_lane_fraction"] = issue["active_lane_fraction"]
+        measured["issue_counters"] = "profiles/issue_workload%d.json" % args.workload
     roofline = {
         "bound": bound,
         "achieved": round(achieved, 4),
@@ -339,6 +441,7 @@ def main():
         "launches_per_step": st["launches"] / steps,
         "flop_per_launch": flop,
         "bytes_per_launch": alg_bytes,
+        "measured": measured,
     }
     stage_summary = {n: {"ms_per_step": round(stages[n]["ms"] / steps, 3),
                          "launches_per_step": stages[n]["launches"] / steps,
@@ -350,9 +453,13 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # the host's CPU share for one GPU is 16 threads, whatever the host's own count
+        visible = len(os.sched_getaffinity(0))
+        threads = min(16, visible)
         cpu, _ = cpu_baseline(scene, params, seed, args.cpu_seconds, threads)
         cpu["gpu_over_cpu"] = round(value / cpu["value"], 2) if cpu["value"] > 0 else None
+        cpu["host_logical_cpus"] = os.cpu_count()
+        cpu["affinity_cpus"] = visible
 
     out = {
         "metric": METRIC,
@@ -376,6 +483,8 @@ def main():
             "schedule": "wavefront" if k_engine == dfa.PT_WAVEFRONT else "megakernel",
             "autotune_passes": tune_passes,
             "rays_per_step": c["rays"] / steps,
+            "traced_mrays_s": round((c["rays"] - stages.get("shade", {"counters": {"rays": 0}})["counters"]["rays"])
+                                    / wall / 1e6, 3),
             "samples_per_s": round(c["samples"] / wall, 1),
         },
         "roofline": roofline,

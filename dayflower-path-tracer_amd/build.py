@@ -29,7 +29,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall"]
 
 SCENE_SOURCES = ["scene/df_scene.cpp", "scene/df_configs.cpp", "scene/df_obj.cpp", "scene/df_scene_capi.cpp"]
-HIP_SOURCES = ["df_kernels.hip", "df_wavefront.hip", "df_adjoint.hip", "df_runtime.hip"]
+HIP_SOURCES = ["df_kernels.hip", "df_wavefront.hip", "df_adjoint.hip", "df_runtime.hip", "df_calibrate.hip"]
 
 
 def run(cmd):

@@ -375,11 +375,15 @@ __device__ __forceinline__ float isect_triangle(float ox, float oy, float oz, fl
 // reference's step-by-step march returns, evaluating far fewer heights:
 //
 // * the march positions tc_k are the same for every ray (`tc`, `n`);
-// * g(tc) = py - h is Lipschitz in tc with constant L = |dx|+|dy|+|dz|
-//   (|dh/dx|, |dh/dz| <= 1), so after a step with gap g no step with
+// * g(tc) = py - h is Lipschitz in tc with constant L = |dy| + sqrt(dx^2 +
+//   dz^2): dg/dtc = dy - (dx cos px sin pz + dz sin px cos pz), and by
+//   Cauchy-Schwarz the bracket is at most sqrt(dx^2 + dz^2) times
+//   sqrt(cos^2 px sin^2 pz + sin^2 px cos^2 pz) <= 1; L is rounded up by a
+//   relative 2^-20.  So after a step with gap g no step with
 //   tc_j <= tc + (g - margin) / L can cross; `margin` bounds the float
 //   rounding of py and h at both steps (positions carry ~2^-23 relative
-//   error, sin ~1 ulp), with a wide safety factor;
+//   error, sin ~1 ulp; it keeps the coarser |dx|+|dy|+|dz| for the
+//   positions' error), with a wide safety factor;
 // * a crossing at step j returns t >= tc_{j-1} (rounded), so once
 //   tc_{k-1} > closest the terrain cannot win the closest-hit comparison
 //   (tc_{k-1} >= 0.98 tc_k);
@@ -393,8 +397,9 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
                                                unsigned* steps) {
   const float sr = 1.0f / 10.0f;
   const float bx = ox * sr, by = oy * sr, bz = oz * sr;
-  const float L = fabsf(dx) + fabsf(dy) + fabsf(dz);
-  const float margin = 1.0e-5f + 9.5367431640625e-07f * (fabsf(bx) + fabsf(by) + fabsf(bz) + 10.0f * L);
+  const float L1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
+  const float L = (fabsf(dy) + sqrtf(dx * dx + dz * dz)) * 1.00000095367431640625f;
+  const float margin = 1.0e-5f + 9.5367431640625e-07f * (fabsf(bx) + fabsf(by) + fabsf(bz) + 10.0f * L1);
   const float inv_l = 1.0f / L;
   float lh = 0.0f, ly = 0.0f, t = 0.0f;
   int last = -1;

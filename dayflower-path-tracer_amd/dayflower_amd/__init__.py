@@ -17,6 +17,7 @@ from .renderer import (  # noqa: F401
     PT_MEGAKERNEL,
     PT_WAVEFRONT,
     device_count,
+    measure_roof,
 )
 from .scene import (  # noqa: F401
     CompiledScene,

@@ -54,6 +54,19 @@ def device_count():
     return n.value
 
 
+ROOFS = {"valu_fp32": 1, "l2_read": 2, "mall_read": 3, "hbm_read": 4}
+
+
+def measure_roof(kind, device=0):
+    """df_measure_roof: the sustained rate of one microbenchmark on `device` (FLOP/s for "valu_fp32", bytes/s
+    for "l2_read", "mall_read", "hbm_read") and the work of one timed launch."""
+    rate, work = ctypes.c_double(), ctypes.c_double()
+    rc = _ffi.hip_lib().df_measure_roof(int(device), ROOFS[kind], ctypes.byref(rate), ctypes.byref(work))
+    if rc != _ffi.DF_OK:
+        raise HIPError(rc, "df_measure_roof(%s) failed" % kind)
+    return rate.value, work.value
+
+
 class HIPRendererKernel:
     """Renderer bound to one compiled scene on one HIP device."""
 

@@ -190,6 +190,16 @@ int df_batch_execute(df_batch* batch);
 int df_batch_last_execute_ms(df_batch* batch, float* ms);
 const char* df_batch_last_error(df_batch* batch);
 
+/* ---- Measured roofs -------------------------------------------------------
+ * (no counterpart in the reference: measurement only.)  Runs one
+ * microbenchmark on `device` and returns its sustained rate: FLOP/s of
+ * single (unpacked) FP32 fused multiply-adds, or bytes/s read from the L2
+ * caches, from the Infinity Cache (MALL), or from HBM.  work_per_launch (may
+ * be NULL) receives the FLOP or bytes of one timed launch.  Allocates and
+ * frees its own buffers (at most 2 GiB, for DF_ROOF_HBM_READ). */
+enum { DF_ROOF_VALU_FP32 = 1, DF_ROOF_L2_READ = 2, DF_ROOF_MALL_READ = 3, DF_ROOF_HBM_READ = 4 };
+int df_measure_roof(int32_t device, int32_t kind, double* rate, double* work_per_launch);
+
 #ifdef __cplusplus
 }
 #endif
