@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <unordered_map>
@@ -842,6 +843,40 @@ int bfail(df_batch* b, int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return bfail((b), DF_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
+// The batch's work tables for one df_batch_execute: the job table, the
+// (job, 16x16 tile) items grouped by (math mode, renderer) launch, and each
+// group's range of items.
+struct BatchPlan {
+  std::vector<DJob> jobs;
+  std::vector<DBatchItem> items;
+  std::vector<int> group_key;       // math_mode * 8 + renderer_type, one entry per group
+  std::vector<size_t> group_begin;  // items of group g: [group_begin[g], group_begin[g + 1])
+};
+
+// ---- allocation-free batch execution (df_batch_execute_noalloc) ----------
+// The work tables live in one static device array per GPU: the group work
+// heads (zeroed by a fill kernel on every call), the job table, the items.
+// Every call on one GPU shares that GPU's array, so calls that overlap on one
+// GPU would race: the entry point serialises the calls of this process per
+// device and synchronises before it returns.
+constexpr int32_t kNoAllocMaxJobs = 1024;
+constexpr int32_t kNoAllocMaxItems = 262144;
+constexpr int32_t kNoAllocMaxGroups = 64;  // at most 2 math modes x 6 renderers occur
+constexpr size_t kScratchHeadsBytes = 256;
+constexpr size_t kScratchJobsBytes = (kNoAllocMaxJobs * sizeof(DJob) + 255) / 256 * 256;
+constexpr size_t kScratchBytes = kScratchHeadsBytes + kScratchJobsBytes + kNoAllocMaxItems * sizeof(DBatchItem);
+static_assert(kNoAllocMaxGroups * sizeof(int) <= kScratchHeadsBytes, "heads region");
+constexpr int kMaxScratchDevices = 64;
+
+std::mutex g_scratch_setup_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+std::mutex g_scratch_call_mu[kMaxScratchDevices];
+
+__global__ void zero_ints(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 int balloc(df_batch* b, DevBuf& d, size_t bytes) {
   free_buf(d);
   if (bytes == 0) bytes = 4;
@@ -851,6 +886,91 @@ int balloc(df_batch* b, DevBuf& d, size_t bytes) {
     return bfail(b, DF_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
   }
   d.bytes = bytes;
+  return DF_OK;
+}
+
+}  // namespace
+
+// The scratch array of df_batch_execute_noalloc: kScratchBytes (about 4.7 MiB)
+// per GPU, held until the process ends.  Host code never uses this name as an
+// address: scratch_address() looks up each device's copy.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// The device address of `scratch` on `device` (which must be current), looked
+// up once per device with hipGetSymbolAddress.
+int scratch_address(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxScratchDevices) return DF_ENODEVICE;
+  std::lock_guard<std::mutex> lock(g_scratch_setup_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return DF_EHIP;
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[device]);
+  return DF_OK;
+}
+
+// Builds the batch's work tables: per job its scene, camera, sky, parameters
+// and frame; per group (math mode, renderer) its (job, tile) items, most
+// expensive first.  Waits for every job's own stream.
+int batch_plan(df_batch* b, BatchPlan& plan) {
+  const size_t n = b->jobs.size();
+  struct Item {
+    double cost;
+    int group;
+    DBatchItem it;
+  };
+  plan.jobs.assign(n, DJob{});
+  std::vector<Item> items;
+  plan.group_key.clear();
+  for (size_t j = 0; j < n; ++j) {
+    df_ctx* c = b->jobs[j];
+    DF_BHIP(b, hipStreamSynchronize(c->stream));
+    DJob& hj = plan.jobs[j];
+    hj.S = c->ds;
+    hj.C = make_camera(c->camera);
+    hj.K = make_sky(c->sky);
+    hj.P = make_params(c->params, c->width, c->height, b->passes[j], c->clear_pending);
+    apply_diff(c, hj.P);
+    hj.F = frame_of(c);
+    if (b->passes[j] == 0) continue;
+    const int key = c->params.math_mode * 8 + c->params.renderer_type;
+    int g = 0;
+    while (g < static_cast<int>(plan.group_key.size()) && plan.group_key[g] != key) ++g;
+    if (g == static_cast<int>(plan.group_key.size())) plan.group_key.push_back(key);
+    // work estimate of a tile: passes x (primitive loop + BVH depth)
+    const double cost = static_cast<double>(b->passes[j]) *
+                        (16.0 + c->ds.prim_count + 8.0 * std::log2(1.0 + static_cast<double>(c->hs.bvh.size())));
+    const int tw = (c->width + 15) / 16, th = (c->height + 15) / 16;
+    for (int ty = 0; ty < th; ++ty)
+      for (int tx = 0; tx < tw; ++tx) items.push_back({cost, g, {static_cast<int32_t>(j), tx, ty, 0}});
+  }
+  // per group, the most expensive items first (longest-processing-time order)
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
+    return a.group != c.group ? a.group < c.group : a.cost > c.cost;
+  });
+  plan.items.resize(items.size());
+  for (size_t i = 0; i < items.size(); ++i) plan.items[i] = items[i].it;
+  plan.group_begin.assign(plan.group_key.size() + 1, items.size());
+  for (size_t i = items.size(); i-- > 0;) plan.group_begin[items[i].group] = i;
+  return DF_OK;
+}
+
+// One persistent launch per group over the tables already on the device
+// (heads zeroed), timed with the batch's events; waits for the batch.
+int batch_launch(df_batch* b, const BatchPlan& plan, const DJob* djobs, const DBatchItem* ditems, int* dheads) {
+  DF_BHIP(b, hipEventRecord(b->ev0, b->stream));
+  for (size_t g = 0; g < plan.group_key.size(); ++g) {
+    const size_t begin = plan.group_begin[g], end = plan.group_begin[g + 1];
+    DF_BHIP(b, dfk_launch_batch(djobs, ditems + begin, static_cast<int>(end - begin), dheads + g,
+                                plan.group_key[g] / 8, plan.group_key[g] % 8, b->stream));
+  }
+  DF_BHIP(b, hipEventRecord(b->ev1, b->stream));
+  DF_BHIP(b, hipStreamSynchronize(b->stream));
+  b->timed = true;
+  for (df_ctx* c : b->jobs) c->clear_pending = false;
   return DF_OK;
 }
 
@@ -1459,6 +1579,12 @@ int df_batch_create(df_batch** out, int32_t device) {
     df_batch_destroy(b);
     return DF_EHIP;
   }
+  // set-up of the allocation-free entry point: this device's scratch address
+  unsigned char* sp = nullptr;
+  if (scratch_address(device, &sp) != DF_OK) {
+    df_batch_destroy(b);
+    return DF_EHIP;
+  }
   *out = b;
   return DF_OK;
 }
@@ -1515,63 +1641,52 @@ int df_batch_execute(df_batch* b) {
   const size_t n = b->jobs.size();
   if (n == 0) return DF_OK;
   DF_BHIP(b, hipSetDevice(b->device));
-  struct Item {
-    double cost;
-    int group;
-    DBatchItem it;
-  };
-  std::vector<DJob> hj(n);
-  std::vector<Item> items;
-  std::vector<int> group_key;  // math_mode * 8 + renderer_type, one entry per group
-  for (size_t j = 0; j < n; ++j) {
-    df_ctx* c = b->jobs[j];
-    DF_BHIP(b, hipStreamSynchronize(c->stream));
-    hj[j].S = c->ds;
-    hj[j].C = make_camera(c->camera);
-    hj[j].K = make_sky(c->sky);
-    hj[j].P = make_params(c->params, c->width, c->height, b->passes[j], c->clear_pending);
-    apply_diff(c, hj[j].P);
-    hj[j].F = frame_of(c);
-    if (b->passes[j] == 0) continue;
-    const int key = c->params.math_mode * 8 + c->params.renderer_type;
-    int g = 0;
-    while (g < static_cast<int>(group_key.size()) && group_key[g] != key) ++g;
-    if (g == static_cast<int>(group_key.size())) group_key.push_back(key);
-    // work estimate of a tile: passes x (primitive loop + BVH depth)
-    const double cost = static_cast<double>(b->passes[j]) *
-                        (16.0 + c->ds.prim_count + 8.0 * std::log2(1.0 + static_cast<double>(c->hs.bvh.size())));
-    const int tw = (c->width + 15) / 16, th = (c->height + 15) / 16;
-    for (int ty = 0; ty < th; ++ty)
-      for (int tx = 0; tx < tw; ++tx) items.push_back({cost, g, {static_cast<int32_t>(j), tx, ty, 0}});
-  }
-  // per group, the most expensive items first (longest-processing-time order)
-  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& c) {
-    return a.group != c.group ? a.group < c.group : a.cost > c.cost;
-  });
-  std::vector<DBatchItem> hi(items.size());
-  for (size_t i = 0; i < items.size(); ++i) hi[i] = items[i].it;
+  BatchPlan plan;
   int rc;
-  if ((rc = balloc(b, b->djobs, n * sizeof(DJob))) || (rc = balloc(b, b->ditems, hi.size() * sizeof(DBatchItem))) ||
-      (rc = balloc(b, b->dheads, (group_key.size() + 1) * sizeof(int))))
+  if ((rc = batch_plan(b, plan))) return rc;
+  if ((rc = balloc(b, b->djobs, n * sizeof(DJob))) ||
+      (rc = balloc(b, b->ditems, plan.items.size() * sizeof(DBatchItem))) ||
+      (rc = balloc(b, b->dheads, (plan.group_key.size() + 1) * sizeof(int))))
     return rc;
-  DF_BHIP(b, hipMemcpy(b->djobs.p, hj.data(), n * sizeof(DJob), hipMemcpyHostToDevice));
-  if (!hi.empty()) DF_BHIP(b, hipMemcpy(b->ditems.p, hi.data(), hi.size() * sizeof(DBatchItem), hipMemcpyHostToDevice));
-  DF_BHIP(b, hipMemset(b->dheads.p, 0, (group_key.size() + 1) * sizeof(int)));
-  DF_BHIP(b, hipEventRecord(b->ev0, b->stream));
-  size_t begin = 0;
-  for (size_t g = 0; g < group_key.size(); ++g) {
-    size_t end = begin;
-    while (end < items.size() && items[end].group == static_cast<int>(g)) ++end;
-    DF_BHIP(b, dfk_launch_batch(static_cast<const DJob*>(b->djobs.p), static_cast<const DBatchItem*>(b->ditems.p) + begin,
-                                static_cast<int>(end - begin), static_cast<int*>(b->dheads.p) + g, group_key[g] / 8,
-                                group_key[g] % 8, b->stream));
-    begin = end;
-  }
-  DF_BHIP(b, hipEventRecord(b->ev1, b->stream));
-  DF_BHIP(b, hipStreamSynchronize(b->stream));
-  b->timed = true;
-  for (df_ctx* c : b->jobs) c->clear_pending = false;
-  return DF_OK;
+  DF_BHIP(b, hipMemcpy(b->djobs.p, plan.jobs.data(), n * sizeof(DJob), hipMemcpyHostToDevice));
+  if (!plan.items.empty())
+    DF_BHIP(b, hipMemcpy(b->ditems.p, plan.items.data(), plan.items.size() * sizeof(DBatchItem),
+                         hipMemcpyHostToDevice));
+  DF_BHIP(b, hipMemset(b->dheads.p, 0, (plan.group_key.size() + 1) * sizeof(int)));
+  return batch_launch(b, plan, static_cast<const DJob*>(b->djobs.p), static_cast<const DBatchItem*>(b->ditems.p),
+                      static_cast<int*>(b->dheads.p));
+}
+
+int df_batch_execute_noalloc(df_batch* b) {
+  if (!b) return DF_EINVAL;
+  const size_t n = b->jobs.size();
+  if (n == 0) return DF_OK;
+  if (n > static_cast<size_t>(kNoAllocMaxJobs))
+    return bfail(b, DF_EINVAL, "df_batch_execute_noalloc: %zu jobs exceed the maximum of %d", n, kNoAllocMaxJobs);
+  DF_BHIP(b, hipSetDevice(b->device));
+  unsigned char* sp = nullptr;
+  if (scratch_address(b->device, &sp) != DF_OK)
+    return bfail(b, DF_EHIP, "df_batch_execute_noalloc: no scratch address on device %d", b->device);
+  BatchPlan plan;
+  int rc;
+  if ((rc = batch_plan(b, plan))) return rc;
+  if (plan.items.size() > static_cast<size_t>(kNoAllocMaxItems))
+    return bfail(b, DF_EINVAL, "df_batch_execute_noalloc: %zu tiles exceed the maximum of %d", plan.items.size(),
+                 kNoAllocMaxItems);
+  if (plan.group_key.size() > static_cast<size_t>(kNoAllocMaxGroups))
+    return bfail(b, DF_EINVAL, "df_batch_execute_noalloc: %zu launch groups exceed the maximum of %d",
+                 plan.group_key.size(), kNoAllocMaxGroups);
+  std::lock_guard<std::mutex> lock(g_scratch_call_mu[b->device]);
+  int* heads = reinterpret_cast<int*>(sp);
+  DJob* jobs = reinterpret_cast<DJob*>(sp + kScratchHeadsBytes);
+  DBatchItem* items = reinterpret_cast<DBatchItem*>(sp + kScratchHeadsBytes + kScratchJobsBytes);
+  DF_BHIP(b, hipMemcpyAsync(jobs, plan.jobs.data(), n * sizeof(DJob), hipMemcpyHostToDevice, b->stream));
+  if (!plan.items.empty())
+    DF_BHIP(b, hipMemcpyAsync(items, plan.items.data(), plan.items.size() * sizeof(DBatchItem),
+                              hipMemcpyHostToDevice, b->stream));
+  hipLaunchKernelGGL(zero_ints, dim3(1), dim3(kNoAllocMaxGroups), 0, b->stream, heads, kNoAllocMaxGroups);
+  DF_BHIP(b, hipGetLastError());
+  return batch_launch(b, plan, jobs, items, heads);
 }
 
 int df_batch_last_execute_ms(df_batch* b, float* ms) {

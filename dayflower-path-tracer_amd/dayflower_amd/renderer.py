@@ -497,6 +497,11 @@ class HIPBatchRenderer:
     def execute(self):
         self._check(self._lib.df_batch_execute(self._b), "df_batch_execute")
 
+    def executeNoAlloc(self):
+        """df_batch_execute_noalloc: the same renders with no device allocation (work tables in the library's
+        static per-GPU scratch array; at most 1024 scenes and 262144 tiles per call)."""
+        self._check(self._lib.df_batch_execute_noalloc(self._b), "df_batch_execute_noalloc")
+
     def lastExecuteMillis(self):
         ms = ctypes.c_float()
         self._check(self._lib.df_batch_last_execute_ms(self._b, ctypes.byref(ms)), "df_batch_last_execute_ms")

@@ -186,6 +186,19 @@ int df_batch_job(df_batch* batch, int32_t index, df_ctx** ctx);
 /* Renders every scene's passes; returns when the batch is done. */
 int df_batch_execute(df_batch* batch);
 
+/* df_batch_execute without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (serving loops,
+ * code around HIP graph capture, harnesses that forbid hipMalloc): the
+ * batch's job table, tile work list and work heads live in a static device
+ * array of about 4.7 MiB per GPU, set up by df_batch_create and held until
+ * the process ends.  Limits: at most 1024 scenes and 262144 16x16 tiles
+ * (their sum over the scenes) per call, else DF_EINVAL with a message and
+ * nothing rendered.  Results equal df_batch_execute's bit for bit.  Calls
+ * share their GPU's array: this process's calls are serialised per device
+ * and each returns after its batch is done; calls that overlapped on one GPU
+ * would race. */
+int df_batch_execute_noalloc(df_batch* batch);
+
 /* Device time of the last df_batch_execute in milliseconds. */
 int df_batch_last_execute_ms(df_batch* batch, float* ms);
 const char* df_batch_last_error(df_batch* batch);
