@@ -29,6 +29,16 @@ hipError_t dfk_launch_pt_wavefront(const dfk::DScene& s, const dfk::DCamera& c, 
                                    const dfk::DParams& p, const dfk::DFrame& f, const dfk::DPaths& w, int math_mode,
                                    hipStream_t stream, const dfk_marks* marks = nullptr);
 
+// Queries the wavefront kernels' persistent grid sizes (done once; call before
+// capturing a pass into a graph).
+void dfk_prepare_wavefront();
+
+// One pass of the same (the launch sequence a frame loop captures in a graph):
+// `clear` overwrites the film, `last` ends with the image pass.
+hipError_t dfk_launch_pt_wavefront_pass(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k,
+                                        const dfk::DParams& p, const dfk::DFrame& f, const dfk::DPaths& w,
+                                        int math_mode, hipStream_t stream, int clear, int last);
+
 // Differentiable path tracer (df_adjoint.hip): dLoss/dfilm weights and the
 // loss of a film against a target, then the path-replay adjoint.
 hipError_t dfk_launch_loss(const float* film, const float* target, float* weight, int n, float scale, double* loss,

@@ -40,6 +40,19 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// One launch sequence captured from a stream and instantiated, with the bytes
+// of the by-value kernel arguments it was captured with: it is replayed while
+// they are unchanged and captured again when they change.
+struct GraphCache {
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned char> key;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key.clear();
+  }
+};
+
 // Host copy of the arrays (kept for re-validation and re-packing after
 // transform / surface updates).
 struct HostScene {
@@ -72,6 +85,11 @@ struct df_ctx {
   // wavefront path state (allocated on first use for the frame size)
   DevBuf path_f4[12], path_q[4], path_ctl;
   dfk::DPaths paths{};
+
+  // frame-loop graphs: one wavefront pass per (clear, last) flag pair, the
+  // megakernel launch (df_set_graphs; off while stage timing is on)
+  bool graphs = true;
+  GraphCache g_wave[2][2], g_mega;
 
   // path-tracer schedule chosen by the autotuner for DF_PT_AUTO (0: not yet)
   int tuned_engine = 0;
@@ -115,6 +133,41 @@ int fail(df_ctx* c, int code, const char* fmt, ...) {
   va_end(ap);
   if (c) c->err = buf;
   return code;
+}
+
+template <class T>
+void key_add(std::vector<unsigned char>& k, const T& v) {
+  const unsigned char* p = reinterpret_cast<const unsigned char*>(&v);
+  k.insert(k.end(), p, p + sizeof(T));
+}
+
+// Enqueues `enqueue`'s launches on `stream` through the graph cache `g`:
+// replays the instantiated graph when `key` equals the one it was captured
+// with, else captures `enqueue` into a new graph and replays that.  No
+// memset or allocation may be enqueued (the launch wrappers enqueue kernels
+// only).  Returns the first HIP error.
+template <class F>
+hipError_t graph_launch(GraphCache& g, const std::vector<unsigned char>& key, hipStream_t stream, F&& enqueue) {
+  if (!g.exec || g.key != key) {
+    g.reset();
+    hipError_t e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    const hipError_t le = enqueue();
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(stream, &graph);
+    if (le != hipSuccess || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return le != hipSuccess ? le : e;
+    }
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g.exec = nullptr;
+      return e;
+    }
+    g.key = key;
+  }
+  return hipGraphLaunch(g.exec, stream);
 }
 
 int hip_fail(df_ctx* c, hipError_t e, const char* what) {
@@ -823,6 +876,9 @@ struct df_batch {
   std::vector<df_ctx*> jobs;
   std::vector<int32_t> passes;
   DevBuf djobs, ditems, dheads;
+  // the group launches of the last plan as a graph (df_batch_set_graphs)
+  bool graphs = true;
+  GraphCache g_launch;
 };
 
 namespace {
@@ -877,9 +933,12 @@ __global__ void zero_ints(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// Makes d hold at least `bytes` (kept when it already does, so that the
+// tables of a repeated batch stay at the addresses its graph was captured with).
 int balloc(df_batch* b, DevBuf& d, size_t bytes) {
-  free_buf(d);
   if (bytes == 0) bytes = 4;
+  if (d.p && d.bytes >= bytes) return DF_OK;
+  free_buf(d);
   const hipError_t e = hipMalloc(&d.p, bytes);
   if (e != hipSuccess) {
     d.p = nullptr;
@@ -961,11 +1020,32 @@ int batch_plan(df_batch* b, BatchPlan& plan) {
 // One persistent launch per group over the tables already on the device
 // (heads zeroed), timed with the batch's events; waits for the batch.
 int batch_launch(df_batch* b, const BatchPlan& plan, const DJob* djobs, const DBatchItem* ditems, int* dheads) {
+  auto enqueue = [&]() -> hipError_t {
+    for (size_t g = 0; g < plan.group_key.size(); ++g) {
+      const size_t begin = plan.group_begin[g], end = plan.group_begin[g + 1];
+      const hipError_t e = dfk_launch_batch(djobs, ditems + begin, static_cast<int>(end - begin), dheads + g,
+                                            plan.group_key[g] / 8, plan.group_key[g] % 8, b->stream);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  };
   DF_BHIP(b, hipEventRecord(b->ev0, b->stream));
-  for (size_t g = 0; g < plan.group_key.size(); ++g) {
-    const size_t begin = plan.group_begin[g], end = plan.group_begin[g + 1];
-    DF_BHIP(b, dfk_launch_batch(djobs, ditems + begin, static_cast<int>(end - begin), dheads + g,
-                                plan.group_key[g] / 8, plan.group_key[g] % 8, b->stream));
+  if (b->graphs) {
+    // the persistent grid sizes are queried (n_items 0: no launch) before any capture
+    for (size_t g = 0; g < plan.group_key.size(); ++g)
+      DF_BHIP(b, dfk_launch_batch(djobs, ditems, 0, dheads, plan.group_key[g] / 8, plan.group_key[g] % 8, b->stream));
+    std::vector<unsigned char> key;
+    key_add(key, djobs);
+    key_add(key, ditems);
+    key_add(key, dheads);
+    for (size_t g = 0; g < plan.group_key.size(); ++g) {
+      key_add(key, plan.group_key[g]);
+      key_add(key, plan.group_begin[g]);
+    }
+    key_add(key, plan.items.size());
+    DF_BHIP(b, graph_launch(b->g_launch, key, b->stream, enqueue));
+  } else {
+    DF_BHIP(b, enqueue());
   }
   DF_BHIP(b, hipEventRecord(b->ev1, b->stream));
   DF_BHIP(b, hipStreamSynchronize(b->stream));
@@ -1060,6 +1140,9 @@ void df_destroy(df_ctx* c) {
   for (DevBuf* b : {&c->diff_surf_albedo, &c->diff_surf_emission, &c->diff_albedo, &c->diff_emission, &c->diff_skies,
                     &c->diff_weight, &c->diff_grad, &c->diff_loss, &c->diff_seeds})
     free_buf(*b);
+  for (auto& row : c->g_wave)
+    for (GraphCache& g : row) g.reset();
+  c->g_mega.reset();
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : c->tune_ev)
     if (e) (void)hipEventDestroy(e);
@@ -1271,13 +1354,46 @@ int df_execute(df_ctx* c, int32_t passes) {
   const dfk_marks* mk = c->stage_timing ? &marks : nullptr;
   c->mark_error = hipSuccess;
   // `n` passes with one schedule; the film is cleared before the first if `clear`
+  // through the frame-loop graphs unless the launches are timed one by one
+  const bool graph = c->graphs && !mk;
   auto run = [&](bool wave, int n, bool clear) -> int {
     DParams par = make_params(c->params, c->width, c->height, n, clear);
     apply_diff(c, par);
-    if (wave) {
-      DF_HIP(c, dfk_launch_pt_wavefront(c->ds, cam, sky, par, f, c->paths, c->params.math_mode, c->stream, mk));
+    const int math = c->params.math_mode;
+    if (wave && graph) {
+      // one graph per pass; the wavefront kernels read neither passes nor clear_first
+      dfk_prepare_wavefront();
+      DParams p1 = par;
+      p1.passes = 1;
+      for (int pass = 0; pass < n; ++pass) {
+        const int cl = pass == 0 && clear ? 1 : 0, last = pass == n - 1 ? 1 : 0;
+        p1.clear_first = cl;
+        std::vector<unsigned char> key;
+        key_add(key, c->ds);
+        key_add(key, cam);
+        key_add(key, sky);
+        key_add(key, p1);
+        key_add(key, f);
+        key_add(key, c->paths);
+        key_add(key, math);
+        DF_HIP(c, graph_launch(c->g_wave[cl][last], key, c->stream, [&] {
+          return dfk_launch_pt_wavefront_pass(c->ds, cam, sky, p1, f, c->paths, math, c->stream, cl, last);
+        }));
+      }
+    } else if (wave) {
+      DF_HIP(c, dfk_launch_pt_wavefront(c->ds, cam, sky, par, f, c->paths, math, c->stream, mk));
+    } else if (graph) {
+      std::vector<unsigned char> key;
+      key_add(key, c->ds);
+      key_add(key, cam);
+      key_add(key, sky);
+      key_add(key, par);
+      key_add(key, f);
+      key_add(key, math);
+      DF_HIP(c, graph_launch(c->g_mega, key, c->stream,
+                             [&] { return dfk_launch_render(c->ds, cam, sky, par, f, math, c->stream, nullptr); }));
     } else {
-      DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, c->params.math_mode, c->stream, mk));
+      DF_HIP(c, dfk_launch_render(c->ds, cam, sky, par, f, math, c->stream, mk));
     }
     return DF_OK;
   };
@@ -1406,6 +1522,19 @@ int df_reset_counters(df_ctx* c) {
   for (int st = 0; st < DF_STAGE_COUNT; ++st) {
     c->stage_ms[st] = 0.0;
     c->stage_launches[st] = 0;
+  }
+  return DF_OK;
+}
+
+int df_set_graphs(df_ctx* c, int32_t enabled) {
+  if (!c) return DF_EINVAL;
+  DF_HIP(c, hipSetDevice(c->device));
+  DF_HIP(c, hipStreamSynchronize(c->stream));
+  c->graphs = enabled != 0;
+  if (!c->graphs) {
+    for (auto& row : c->g_wave)
+      for (GraphCache& g : row) g.reset();
+    c->g_mega.reset();
   }
   return DF_OK;
 }
@@ -1594,6 +1723,7 @@ void df_batch_destroy(df_batch* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   for (df_ctx* c : b->jobs) df_destroy(c);
+  b->g_launch.reset();
   free_buf(b->djobs);
   free_buf(b->ditems);
   free_buf(b->dheads);
@@ -1687,6 +1817,15 @@ int df_batch_execute_noalloc(df_batch* b) {
   hipLaunchKernelGGL(zero_ints, dim3(1), dim3(kNoAllocMaxGroups), 0, b->stream, heads, kNoAllocMaxGroups);
   DF_BHIP(b, hipGetLastError());
   return batch_launch(b, plan, jobs, items, heads);
+}
+
+int df_batch_set_graphs(df_batch* b, int32_t enabled) {
+  if (!b) return DF_EINVAL;
+  DF_BHIP(b, hipSetDevice(b->device));
+  DF_BHIP(b, hipStreamSynchronize(b->stream));
+  b->graphs = enabled != 0;
+  if (!b->graphs) b->g_launch.reset();
+  return DF_OK;
 }
 
 int df_batch_last_execute_ms(df_batch* b, float* ms) {

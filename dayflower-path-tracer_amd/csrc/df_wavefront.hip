@@ -345,13 +345,27 @@ int resident_blocks(Kernel k) {
   return cus * per_cu;
 }
 
+// Persistent grid sizes of the queue kernels, queried once per math mode (before
+// any graph capture: the occupancy queries are not stream work).
+struct Grids {
+  int trace, shade, escape, normals;
+};
+
 template <int M>
-hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
-                            const DPaths& w, hipStream_t stream, const dfk_marks* marks) {
-  static const int g_trace = resident_blocks(wf_trace<M, false>);
-  static const int g_shade = resident_blocks(wf_shade<M>);
-  static const int g_escape = resident_blocks(wf_trace<M, true>);
-  static const int g_normals = resident_blocks(wf_normals<M>);
+const Grids& grids() {
+  static const Grids g{resident_blocks(wf_trace<M, false>), resident_blocks(wf_shade<M>),
+                       resident_blocks(wf_trace<M, true>), resident_blocks(wf_normals<M>)};
+  return g;
+}
+
+// One pass of the wavefront path tracer: every launch of one sample per pixel,
+// in stream order.  `clear`: the film pass overwrites (the clear flag);
+// `last`: the film pass ends with the image pass.
+template <int M>
+hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
+                                 const DPaths& w, hipStream_t stream, const dfk_marks* marks, int clear, int last) {
+  const Grids& g = grids<M>();
+  const int g_trace = g.trace, g_shade = g.shade, g_escape = g.escape, g_normals = g.normals;
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
@@ -370,7 +384,7 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
   auto mark = [&](int stage, int begin) {
     if (marks) marks->mark(marks->user, stage, begin);
   };
-  for (int pass = 0; pass < p.passes; ++pass) {
+  {
     hipLaunchKernelGGL(wf_reset, dim3((ctl_len + 255) / 256), dim3(256), 0, stream, ctl, ctl_len);
     mark(kStageRaygen, 1);
     hipLaunchKernelGGL(wf_raygen<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w);
@@ -398,11 +412,21 @@ hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, co
                        head(3 * bounces + 1), 0, nullptr);
     mark(kStageEscape, 0);
     mark(kStageFilm, 1);
-    hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w,
-                       pass == 0 && p.clear_first ? 1 : 0, pass == p.passes - 1 ? 1 : 0);
+    hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w, clear, last);
     mark(kStageFilm, 0);
   }
   return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_wavefront(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
+                            const DPaths& w, hipStream_t stream, const dfk_marks* marks) {
+  for (int pass = 0; pass < p.passes; ++pass) {
+    const hipError_t e = launch_wavefront_pass<M>(s, c, k, p, f, w, stream, marks, pass == 0 && p.clear_first ? 1 : 0,
+                                                  pass == p.passes - 1 ? 1 : 0);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace dfk
@@ -412,4 +436,16 @@ hipError_t dfk_launch_pt_wavefront(const dfk::DScene& s, const dfk::DCamera& c, 
                                    hipStream_t stream, const dfk_marks* marks) {
   return math_mode == 1 ? dfk::launch_wavefront<1>(s, c, k, p, f, w, stream, marks)
                         : dfk::launch_wavefront<0>(s, c, k, p, f, w, stream, marks);
+}
+
+void dfk_prepare_wavefront() {
+  (void)dfk::grids<0>();
+  (void)dfk::grids<1>();
+}
+
+hipError_t dfk_launch_pt_wavefront_pass(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k,
+                                        const dfk::DParams& p, const dfk::DFrame& f, const dfk::DPaths& w,
+                                        int math_mode, hipStream_t stream, int clear, int last) {
+  return math_mode == 1 ? dfk::launch_wavefront_pass<1>(s, c, k, p, f, w, stream, nullptr, clear, last)
+                        : dfk::launch_wavefront_pass<0>(s, c, k, p, f, w, stream, nullptr, clear, last);
 }

@@ -178,6 +178,8 @@ HIP_PROTOTYPES = {
     "df_batch_job": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(_P)]),
     "df_batch_execute": (ctypes.c_int, [_P]),
     "df_batch_execute_noalloc": (ctypes.c_int, [_P]),
+    "df_batch_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "df_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "df_batch_last_execute_ms": (ctypes.c_int, [_P, c_float_p]),
     "df_batch_last_error": (ctypes.c_char_p, [_P]),
 }

@@ -199,6 +199,10 @@ class HIPRendererKernel:
         """execute(range) + clearFilmFlags(), ``passes`` times (asynchronous)."""
         self._check(self._lib.df_execute(self._ctx, int(passes)), "df_execute")
 
+    def setGraphs(self, enabled):
+        """df_set_graphs: replay the frame loop's launches from captured HIP graphs (default on)."""
+        self._check(self._lib.df_set_graphs(self._ctx, 1 if enabled else 0), "df_set_graphs")
+
     def synchronize(self):
         self._check(self._lib.df_synchronize(self._ctx), "df_synchronize")
 
@@ -496,6 +500,10 @@ class HIPBatchRenderer:
 
     def execute(self):
         self._check(self._lib.df_batch_execute(self._b), "df_batch_execute")
+
+    def setGraphs(self, enabled):
+        """df_batch_set_graphs: replay the batch's launches from a captured HIP graph (default on)."""
+        self._check(self._lib.df_batch_set_graphs(self._b, 1 if enabled else 0), "df_batch_set_graphs")
 
     def executeNoAlloc(self):
         """df_batch_execute_noalloc: the same renders with no device allocation (work tables in the library's

@@ -109,6 +109,15 @@ int df_execute(df_ctx* ctx, int32_t passes);
  * two on the current frame (no counterpart in the reference). */
 int df_get_engine(df_ctx* ctx, int32_t* engine);
 
+/* Frame-loop graphs (no counterpart in the reference): with graphs on (the
+ * default), df_execute captures the launches of one wavefront pass (one graph
+ * per combination of the film-clear and image-pass flags) or the megakernel
+ * launch into a HIP graph and replays it while the scene, camera, sky,
+ * parameters and frame are unchanged; any change is captured anew on the
+ * next df_execute.  Results are the same bits either way.  Launches timed by
+ * df_set_stage_timing are not captured. */
+int df_set_graphs(df_ctx* ctx, int32_t enabled);
+
 /* Waits for the handle's stream. */
 int df_synchronize(df_ctx* ctx);
 
@@ -198,6 +207,10 @@ int df_batch_execute(df_batch* batch);
  * and each returns after its batch is done; calls that overlapped on one GPU
  * would race. */
 int df_batch_execute_noalloc(df_batch* batch);
+
+/* Graphs for batches (default on): the group launches of a batch are
+ * captured once and replayed while the batch's work tables keep their size. */
+int df_batch_set_graphs(df_batch* batch, int32_t enabled);
 
 /* Device time of the last df_batch_execute in milliseconds. */
 int df_batch_last_execute_ms(df_batch* batch, float* ms);
