@@ -87,8 +87,9 @@ struct df_ctx {
   dfk::DPaths paths{};
 
   // frame-loop graphs: one wavefront pass per (clear, last) flag pair, the
-  // megakernel launch (df_set_graphs; off while stage timing is on)
-  bool graphs = true;
+  // megakernel launch (df_set_graphs, default off: replaying measured 1-7 %
+  // slower per pass than launching eagerly; off while stage timing is on)
+  bool graphs = false;
   GraphCache g_wave[2][2], g_mega;
 
   // path-tracer schedule chosen by the autotuner for DF_PT_AUTO (0: not yet)
@@ -876,8 +877,9 @@ struct df_batch {
   std::vector<df_ctx*> jobs;
   std::vector<int32_t> passes;
   DevBuf djobs, ditems, dheads;
-  // the group launches of the last plan as a graph (df_batch_set_graphs)
-  bool graphs = true;
+  // the group launches of the last plan as a graph (df_batch_set_graphs,
+  // default off like the renderer's)
+  bool graphs = false;
   GraphCache g_launch;
 };
 

@@ -200,7 +200,7 @@ class HIPRendererKernel:
         self._check(self._lib.df_execute(self._ctx, int(passes)), "df_execute")
 
     def setGraphs(self, enabled):
-        """df_set_graphs: replay the frame loop's launches from captured HIP graphs (default on)."""
+        """df_set_graphs: replay the frame loop's launches from captured HIP graphs (default off)."""
         self._check(self._lib.df_set_graphs(self._ctx, 1 if enabled else 0), "df_set_graphs")
 
     def synchronize(self):
@@ -502,7 +502,7 @@ class HIPBatchRenderer:
         self._check(self._lib.df_batch_execute(self._b), "df_batch_execute")
 
     def setGraphs(self, enabled):
-        """df_batch_set_graphs: replay the batch's launches from a captured HIP graph (default on)."""
+        """df_batch_set_graphs: replay the batch's launches from a captured HIP graph (default off)."""
         self._check(self._lib.df_batch_set_graphs(self._b, 1 if enabled else 0), "df_batch_set_graphs")
 
     def executeNoAlloc(self):
