@@ -144,6 +144,11 @@ struct DParams {
   int32_t width, height;
   int32_t passes;
   int32_t clear_first;
+  // wavefront schedule: 1 = the trace kernel bins each closest-hit result by
+  // its hit surface's material and the shade kernel drains the bins one by one
+  int32_t shade_binned;
+  // megakernel schedule: 0 = a wave covers a 16x4 strip of its 16x16 tile, 1 = an 8x8 quadrant
+  int32_t wave_tile;
   // sRGB constants (image4j RGBColorSpace.SRGB)
   float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
   // differentiable mode (diff_on = 1): continuous albedo / emission
@@ -218,6 +223,7 @@ struct DPaths {
   int32_t* queue[2];  // live paths of even / odd bounces (kQueueStripes stripes of n entries)
   int32_t* shadow_queue;
   int32_t* noise_queue;  // a bounce's paths whose hit surface has noise normal mapping
+  int32_t* shade_bins;   // kShadeBins bins of kQueueStripes stripes of n entries (binned shading)
   float4* noise;         // their three normal-map fBm values (x, y, z)
   int32_t* ctl;       // per-pass queue counts and work heads (groups of kCtlGroup ints)
   int64_t n;          // slots; also the capacity of one queue stripe
@@ -233,13 +239,18 @@ constexpr int kQueueStripes = 16;
 constexpr int kLineInts = 16;                          // one 64-byte line
 constexpr int kCtlGroup = kQueueStripes * kLineInts;  // one count or head per stripe
 
+// Shade bins of the binned schedule: 0 the paths that hit nothing, 1..5 the
+// hits on a surface of that material (ClearCoat, Glass, Lambertian, Phong,
+// Reflection), 6 the hits on a surface with any other material value.
+constexpr int kShadeBins = 7;
+
 // Length of DPaths::ctl for a maximum depth, in ints: B + 1 live-path counts,
-// B + 1 shadow counts (the last stays zero), 3B + 4 work heads and per
-// bounce the noise queue's count and work head, B = max(1, depth), each a
-// group of per-stripe lines.
+// B + 1 shadow counts (the last stays zero), 3B + 4 work heads, per bounce the
+// noise queue's count and work head, then per bounce the shade bins' counts
+// and work heads, B = max(1, depth), each a group of per-stripe lines.
 __host__ __device__ inline int wavefront_ctl_len(int depth_max) {
   const int b = depth_max > 1 ? depth_max : 1;
-  return (7 * b + 6) * kCtlGroup;
+  return (7 * b + 6 + 2 * kShadeBins * b) * kCtlGroup;
 }
 
 }  // namespace dfk

@@ -90,12 +90,23 @@ __device__ __forceinline__ void stage_perm_2d(int* perm) {
 // Occupancy hint: four waves per SIMD (spilling some registers); without it
 // the allocation leaves two, and the 512x512 mesh workload's pass took 4.55
 // ms instead of 3.70 (five: 3.77, six: 3.88, eight: 4.42).
+//
+// P.wave_tile chooses the pixels of a workgroup's four waves: 0 = rows (a
+// wave covers a 16x4 strip), 1 = squares (a wave covers an 8x8 quadrant, a
+// more coherent set of primary rays); either way every pixel of the 16x16
+// tile gets one work-item and the same arithmetic.
 template <int M, int R>
 __global__ __launch_bounds__(256, 4) void render_kernel(DScene S, DCamera C, DSky K, DParams P, DFrame F) {
   __shared__ int perm[512];
   stage_perm_2d(perm);
-  const int x = blockIdx.x * 16 + threadIdx.x;
-  const int y = blockIdx.y * 16 + threadIdx.y;
+  int tx = threadIdx.x, ty = threadIdx.y;
+  if (P.wave_tile == 1) {
+    const int t = threadIdx.y * 16 + threadIdx.x, wave = t >> 6, lane = t & 63;
+    tx = (wave & 1) * 8 + (lane & 7);
+    ty = (wave >> 1) * 8 + (lane >> 3);
+  }
+  const int x = blockIdx.x * 16 + tx;
+  const int y = blockIdx.y * 16 + ty;
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (x < P.width && y < P.height) render_pixel<M, R>(S, C, K, P, F, perm, x, y, cnt);
   flush_counters(cnt, F.counters, kStageMega);

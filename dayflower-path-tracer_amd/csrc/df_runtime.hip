@@ -40,6 +40,17 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// The path tracer's schedules the autotuner times for DF_PT_AUTO (every one
+// gives the same bits): the megakernel with strip or square waves, the
+// wavefront with one shade queue or with material bins.
+constexpr int kTuneCandidates = 4;
+constexpr df_schedule kCandidates[kTuneCandidates] = {
+    {DF_PT_MEGAKERNEL, DF_TILE_ROWS, 0, 0},
+    {DF_PT_MEGAKERNEL, DF_TILE_SQUARE, 0, 0},
+    {DF_PT_WAVEFRONT, DF_TILE_ROWS, 0, 0},
+    {DF_PT_WAVEFRONT, DF_TILE_ROWS, 1, 0},
+};
+
 // One launch sequence captured from a stream and instantiated, with the bytes
 // of the by-value kernel arguments it was captured with: it is replayed while
 // they are unchanged and captured again when they change.
@@ -82,8 +93,9 @@ struct df_ctx {
   DevBuf film, film_samples, image_float, image_bgra, prim_offsets, seeds, counters;
   bool clear_pending = false;
 
-  // wavefront path state (allocated on first use for the frame size)
-  DevBuf path_f4[12], path_q[4], path_ctl;
+  // wavefront path state (allocated on first use for the frame size); the
+  // shade bins only once a binned schedule runs
+  DevBuf path_f4[12], path_q[4], path_ctl, path_bins;
   dfk::DPaths paths{};
 
   // frame-loop graphs: one wavefront pass per (clear, last) flag pair, the
@@ -92,10 +104,14 @@ struct df_ctx {
   bool graphs = false;
   GraphCache g_wave[2][2], g_mega;
 
-  // path-tracer schedule chosen by the autotuner for DF_PT_AUTO (0: not yet)
-  int tuned_engine = 0;
-  int tune_runs[2] = {0, 0};      // megakernel, wavefront passes timed
-  float tune_ms[2] = {0.0f, 0.0f};  // best time of one pass
+  // path-tracer schedule: fixed by df_set_schedule (sched_fixed), else for
+  // DF_PT_AUTO the autotuner's choice once `tuned`, else the default
+  // schedule of params.pt_engine (current_schedule)
+  bool sched_fixed = false;
+  bool tuned = false;
+  df_schedule sched{};
+  int tune_runs[kTuneCandidates] = {};     // passes timed per candidate
+  float tune_ms[kTuneCandidates] = {};     // best time of one pass
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
 
   // differentiable mode
@@ -738,63 +754,81 @@ void free_paths(df_ctx* c) {
   for (DevBuf& b : c->path_f4) free_buf(b);
   for (DevBuf& b : c->path_q) free_buf(b);
   free_buf(c->path_ctl);
+  free_buf(c->path_bins);
   c->paths = dfk::DPaths{};
 }
 
 // Path state of the wavefront path tracer for the current frame size and a
 // maximum depth: per pixel 12 float4 records and an entry in every stripe of
 // 4 queues (448 bytes).
-int ensure_paths(df_ctx* c, int depth_max) {
+int ensure_paths(df_ctx* c, int depth_max, bool bins) {
   const size_t n = pixels(c);
   const int ctl_len = dfk::wavefront_ctl_len(depth_max);
-  if (c->paths.n == static_cast<int64_t>(n) && c->paths.ctl_len >= ctl_len) return DF_OK;
-  DF_HIP(c, hipStreamSynchronize(c->stream));  // earlier launches may still read the old buffers
-  free_paths(c);
   int rc;
-  for (DevBuf& b : c->path_f4)
-    if ((rc = alloc_buf(c, b, n * sizeof(float4)))) return rc;
-  for (DevBuf& b : c->path_q)
-    if ((rc = alloc_buf(c, b, n * dfk::kQueueStripes * sizeof(int32_t)))) return rc;
-  const int ctl_alloc = ctl_len > 64 * dfk::kCtlGroup ? ctl_len : 64 * dfk::kCtlGroup;
-  if ((rc = alloc_buf(c, c->path_ctl, static_cast<size_t>(ctl_alloc) * sizeof(int32_t)))) return rc;
-  dfk::DPaths& w = c->paths;
-  float4** f4[12] = {&w.ray_o, &w.ray_d, &w.thru, &w.rad, &w.hit, &w.sh_o, &w.sh_d, &w.rad_lit, &w.sample,
-                     &w.ray0_o, &w.ray0_d, &w.noise};
-  for (int i = 0; i < 12; ++i) *f4[i] = static_cast<float4*>(c->path_f4[i].p);
-  w.queue[0] = static_cast<int32_t*>(c->path_q[0].p);
-  w.queue[1] = static_cast<int32_t*>(c->path_q[1].p);
-  w.shadow_queue = static_cast<int32_t*>(c->path_q[2].p);
-  w.noise_queue = static_cast<int32_t*>(c->path_q[3].p);
-  w.ctl = static_cast<int32_t*>(c->path_ctl.p);
-  w.n = static_cast<int64_t>(n);
-  w.ctl_len = ctl_alloc;
+  if (c->paths.n != static_cast<int64_t>(n) || c->paths.ctl_len < ctl_len) {
+    DF_HIP(c, hipStreamSynchronize(c->stream));  // earlier launches may still read the old buffers
+    free_paths(c);
+    for (DevBuf& b : c->path_f4)
+      if ((rc = alloc_buf(c, b, n * sizeof(float4)))) return rc;
+    for (DevBuf& b : c->path_q)
+      if ((rc = alloc_buf(c, b, n * dfk::kQueueStripes * sizeof(int32_t)))) return rc;
+    const int ctl_alloc = ctl_len > 64 * dfk::kCtlGroup ? ctl_len : 64 * dfk::kCtlGroup;
+    if ((rc = alloc_buf(c, c->path_ctl, static_cast<size_t>(ctl_alloc) * sizeof(int32_t)))) return rc;
+    dfk::DPaths& w = c->paths;
+    float4** f4[12] = {&w.ray_o, &w.ray_d, &w.thru, &w.rad, &w.hit, &w.sh_o, &w.sh_d, &w.rad_lit, &w.sample,
+                       &w.ray0_o, &w.ray0_d, &w.noise};
+    for (int i = 0; i < 12; ++i) *f4[i] = static_cast<float4*>(c->path_f4[i].p);
+    w.queue[0] = static_cast<int32_t*>(c->path_q[0].p);
+    w.queue[1] = static_cast<int32_t*>(c->path_q[1].p);
+    w.shadow_queue = static_cast<int32_t*>(c->path_q[2].p);
+    w.noise_queue = static_cast<int32_t*>(c->path_q[3].p);
+    w.ctl = static_cast<int32_t*>(c->path_ctl.p);
+    w.n = static_cast<int64_t>(n);
+    w.ctl_len = ctl_alloc;
+  }
+  if (bins && !c->path_bins.p) {
+    DF_HIP(c, hipStreamSynchronize(c->stream));
+    if ((rc = alloc_buf(c, c->path_bins, n * dfk::kShadeBins * dfk::kQueueStripes * sizeof(int32_t)))) return rc;
+    c->paths.shade_bins = static_cast<int32_t*>(c->path_bins.p);
+  }
   return DF_OK;
 }
 
-// The path tracer's schedule for this frame.
-bool use_wavefront(const df_ctx* c) {
-  if (c->params.renderer_type != DF_RENDERER_PATH_TRACER) return false;
-  if (c->params.pt_engine == DF_PT_WAVEFRONT) return true;
-  if (c->params.pt_engine == DF_PT_MEGAKERNEL) return false;
-  return c->tuned_engine == DF_PT_WAVEFRONT;
-}
-
-// DF_PT_AUTO chooses by measurement: the first passes of a frame alternate
-// the two schedules, one pass each, until each has run kTunePasses times, and
-// the one with the shorter best pass is kept (both give the same bits, so
-// splitting passes between them changes nothing in the result).  Changing
-// the scene, the resolution or the parameters starts over.
+// DF_PT_AUTO chooses by measurement: the first passes of a frame time each
+// schedule of kCandidates in turn, one pass at a time, until each has run
+// kTunePasses times, and the one with the shortest best pass is kept (every
+// schedule gives the same bits, so passes split between them change nothing
+// in the result).  Changing the scene, the resolution or the parameters
+// starts over.
 constexpr int kTunePasses = 2;
 
 bool tuning(const df_ctx* c) {
   return c->params.renderer_type == DF_RENDERER_PATH_TRACER && c->params.pt_engine == DF_PT_AUTO &&
-         c->tuned_engine == 0;
+         !c->sched_fixed && !c->tuned;
 }
 
 void reset_tuning(df_ctx* c) {
-  c->tuned_engine = 0;
-  c->tune_runs[0] = c->tune_runs[1] = 0;
-  c->tune_ms[0] = c->tune_ms[1] = 0.0f;
+  c->tuned = false;
+  for (int k = 0; k < kTuneCandidates; ++k) {
+    c->tune_runs[k] = 0;
+    c->tune_ms[k] = 0.0f;
+  }
+}
+
+// The schedule of the next passes once tuning is over: the fixed one, the
+// tuned one, or the default of the engine params.pt_engine names (the other
+// renderers run as the megakernel).
+df_schedule current_schedule(const df_ctx* c) {
+  if (c->sched_fixed || (c->tuned && c->params.pt_engine == DF_PT_AUTO)) return c->sched;
+  df_schedule d{DF_PT_MEGAKERNEL, DF_TILE_ROWS, 0, 0};
+  if (c->params.renderer_type == DF_RENDERER_PATH_TRACER && c->params.pt_engine == DF_PT_WAVEFRONT)
+    d.engine = DF_PT_WAVEFRONT;
+  return d;
+}
+
+// The path tracer's schedule for this frame.
+bool use_wavefront(const df_ctx* c) {
+  return c->params.renderer_type == DF_RENDERER_PATH_TRACER && current_schedule(c).engine == DF_PT_WAVEFRONT;
 }
 
 hipEvent_t take_event(df_ctx* c) {
@@ -909,6 +943,7 @@ struct BatchPlan {
   std::vector<DBatchItem> items;
   std::vector<int> group_key;       // math_mode * 8 + renderer_type, one entry per group
   std::vector<size_t> group_begin;  // items of group g: [group_begin[g], group_begin[g + 1])
+  std::vector<size_t> wave_jobs;    // jobs rendered on their own wavefront schedule instead
 };
 
 // ---- allocation-free batch execution (df_batch_execute_noalloc) ----------
@@ -975,8 +1010,12 @@ int scratch_address(int device, unsigned char** out) {
 
 // Builds the batch's work tables: per job its scene, camera, sky, parameters
 // and frame; per group (math mode, renderer) its (job, tile) items, most
-// expensive first.  Waits for every job's own stream.
-int batch_plan(df_batch* b, BatchPlan& plan) {
+// expensive first.  With `allow_wave`, a path-tracer job whose own schedule is
+// the wavefront (fixed with df_set_schedule, given by its pt_engine, or chosen
+// by its autotuner: large mesh scenes) is left out of the items and rendered
+// on that schedule, its path state allocated here.  Waits for every job's own
+// stream.
+int batch_plan(df_batch* b, BatchPlan& plan, bool allow_wave) {
   const size_t n = b->jobs.size();
   struct Item {
     double cost;
@@ -986,6 +1025,7 @@ int batch_plan(df_batch* b, BatchPlan& plan) {
   plan.jobs.assign(n, DJob{});
   std::vector<Item> items;
   plan.group_key.clear();
+  plan.wave_jobs.clear();
   for (size_t j = 0; j < n; ++j) {
     df_ctx* c = b->jobs[j];
     DF_BHIP(b, hipStreamSynchronize(c->stream));
@@ -997,6 +1037,12 @@ int batch_plan(df_batch* b, BatchPlan& plan) {
     apply_diff(c, hj.P);
     hj.F = frame_of(c);
     if (b->passes[j] == 0) continue;
+    if (allow_wave && use_wavefront(c)) {
+      if (const int rc = ensure_paths(c, c->params.pt_depth_maximum, current_schedule(c).shade_binned != 0))
+        return bfail(b, rc, "job %zu: %s", j, df_last_error(c));
+      plan.wave_jobs.push_back(j);
+      continue;
+    }
     const int key = c->params.math_mode * 8 + c->params.renderer_type;
     int g = 0;
     while (g < static_cast<int>(plan.group_key.size()) && plan.group_key[g] != key) ++g;
@@ -1048,6 +1094,13 @@ int batch_launch(df_batch* b, const BatchPlan& plan, const DJob* djobs, const DB
     DF_BHIP(b, graph_launch(b->g_launch, key, b->stream, enqueue));
   } else {
     DF_BHIP(b, enqueue());
+  }
+  for (const size_t j : plan.wave_jobs) {  // on their own (wavefront) schedule, on the batch's stream
+    df_ctx* c = b->jobs[j];
+    DParams par = plan.jobs[j].P;
+    par.shade_binned = current_schedule(c).shade_binned;
+    DF_BHIP(b, dfk_launch_pt_wavefront(c->ds, plan.jobs[j].C, plan.jobs[j].K, par, plan.jobs[j].F, c->paths,
+                                       c->params.math_mode, b->stream, nullptr));
   }
   DF_BHIP(b, hipEventRecord(b->ev1, b->stream));
   DF_BHIP(b, hipStreamSynchronize(b->stream));
@@ -1351,16 +1404,20 @@ int df_execute(df_ctx* c, int32_t passes) {
   const DSky sky = make_sky(c->sky);
   const DFrame f = frame_of(c);
   const bool may_wave = use_wavefront(c) || tuning(c);
-  if (may_wave && (rc = ensure_paths(c, c->params.pt_depth_maximum))) return rc;
+  const bool may_bin = tuning(c) || (use_wavefront(c) && current_schedule(c).shade_binned);
+  if (may_wave && (rc = ensure_paths(c, c->params.pt_depth_maximum, may_bin))) return rc;
   const dfk_marks marks{stage_mark, c};
   const dfk_marks* mk = c->stage_timing ? &marks : nullptr;
   c->mark_error = hipSuccess;
   // `n` passes with one schedule; the film is cleared before the first if `clear`
   // through the frame-loop graphs unless the launches are timed one by one
   const bool graph = c->graphs && !mk;
-  auto run = [&](bool wave, int n, bool clear) -> int {
+  auto run = [&](const df_schedule& sc, int n, bool clear) -> int {
+    const bool wave = c->params.renderer_type == DF_RENDERER_PATH_TRACER && sc.engine == DF_PT_WAVEFRONT;
     DParams par = make_params(c->params, c->width, c->height, n, clear);
     apply_diff(c, par);
+    par.shade_binned = wave && sc.shade_binned ? 1 : 0;
+    par.wave_tile = sc.wave_tile;
     const int math = c->params.math_mode;
     if (wave && graph) {
       // one graph per pass; the wavefront kernels read neither passes nor clear_first
@@ -1404,20 +1461,32 @@ int df_execute(df_ctx* c, int32_t passes) {
   while (done < passes && tuning(c)) {
     for (hipEvent_t& e : c->tune_ev)
       if (!e) DF_HIP(c, hipEventCreate(&e));
-    const int w = c->tune_runs[0] > c->tune_runs[1] ? 1 : 0;
+    int w = 0;  // the candidate timed least often so far (the first of them)
+    for (int k = 1; k < kTuneCandidates; ++k)
+      if (c->tune_runs[k] < c->tune_runs[w]) w = k;
     DF_HIP(c, hipEventRecord(c->tune_ev[0], c->stream));
-    if ((rc = run(w == 1, 1, c->clear_pending && done == 0))) return rc;
+    if ((rc = run(kCandidates[w], 1, c->clear_pending && done == 0))) return rc;
     DF_HIP(c, hipEventRecord(c->tune_ev[1], c->stream));
     DF_HIP(c, hipEventSynchronize(c->tune_ev[1]));
     float ms = 0.0f;
     DF_HIP(c, hipEventElapsedTime(&ms, c->tune_ev[0], c->tune_ev[1]));
     c->tune_ms[w] = c->tune_runs[w] == 0 ? ms : std::min(c->tune_ms[w], ms);
     ++c->tune_runs[w];
-    if (c->tune_runs[0] >= kTunePasses && c->tune_runs[1] >= kTunePasses)
-      c->tuned_engine = c->tune_ms[1] < c->tune_ms[0] ? DF_PT_WAVEFRONT : DF_PT_MEGAKERNEL;
+    if (c->tune_runs[w] >= kTunePasses) {
+      bool all = true;
+      int best = 0;
+      for (int k = 0; k < kTuneCandidates; ++k) {
+        all = all && c->tune_runs[k] >= kTunePasses;
+        if (c->tune_ms[k] < c->tune_ms[best]) best = k;
+      }
+      if (all) {
+        c->sched = kCandidates[best];
+        c->tuned = true;
+      }
+    }
     ++done;
   }
-  if (done < passes && (rc = run(use_wavefront(c), passes - done, c->clear_pending && done == 0))) return rc;
+  if (done < passes && (rc = run(current_schedule(c), passes - done, c->clear_pending && done == 0))) return rc;
   DF_HIP(c, hipEventRecord(c->ev1, c->stream));
   if (c->mark_error != hipSuccess) return hip_fail(c, c->mark_error, "stage timing event");
   c->timed = true;
@@ -1427,8 +1496,31 @@ int df_execute(df_ctx* c, int32_t passes) {
 
 int df_get_engine(df_ctx* c, int32_t* engine) {
   if (!c || !engine) return DF_EINVAL;
-  if (c->params.pt_engine != DF_PT_AUTO) *engine = c->params.pt_engine;
-  else *engine = c->tuned_engine;
+  *engine = tuning(c) ? DF_PT_AUTO : current_schedule(c).engine;
+  return DF_OK;
+}
+
+int df_get_schedule(df_ctx* c, df_schedule* out) {
+  if (!c || !out) return DF_EINVAL;
+  *out = current_schedule(c);
+  if (tuning(c)) out->engine = DF_PT_AUTO;
+  return DF_OK;
+}
+
+int df_set_schedule(df_ctx* c, const df_schedule* s) {
+  if (!c) return DF_EINVAL;
+  if (!s) {
+    c->sched_fixed = false;
+    reset_tuning(c);
+    return DF_OK;
+  }
+  if ((s->engine != DF_PT_MEGAKERNEL && s->engine != DF_PT_WAVEFRONT) ||
+      (s->wave_tile != DF_TILE_ROWS && s->wave_tile != DF_TILE_SQUARE) || (s->shade_binned != 0 && s->shade_binned != 1))
+    return fail(c, DF_EINVAL, "schedule (engine %d, wave_tile %d, shade_binned %d)", s->engine, s->wave_tile,
+                s->shade_binned);
+  c->sched = *s;
+  c->sched.reserved = 0;
+  c->sched_fixed = true;
   return DF_OK;
 }
 
@@ -1775,7 +1867,7 @@ int df_batch_execute(df_batch* b) {
   DF_BHIP(b, hipSetDevice(b->device));
   BatchPlan plan;
   int rc;
-  if ((rc = batch_plan(b, plan))) return rc;
+  if ((rc = batch_plan(b, plan, true))) return rc;
   if ((rc = balloc(b, b->djobs, n * sizeof(DJob))) ||
       (rc = balloc(b, b->ditems, plan.items.size() * sizeof(DBatchItem))) ||
       (rc = balloc(b, b->dheads, (plan.group_key.size() + 1) * sizeof(int))))
@@ -1801,7 +1893,7 @@ int df_batch_execute_noalloc(df_batch* b) {
     return bfail(b, DF_EHIP, "df_batch_execute_noalloc: no scratch address on device %d", b->device);
   BatchPlan plan;
   int rc;
-  if ((rc = batch_plan(b, plan))) return rc;
+  if ((rc = batch_plan(b, plan, false))) return rc;  // the persistent batch kernel only: no path state
   if (plan.items.size() > static_cast<size_t>(kNoAllocMaxItems))
     return bfail(b, DF_EINVAL, "df_batch_execute_noalloc: %zu tiles exceed the maximum of %d", plan.items.size(),
                  kNoAllocMaxItems);

@@ -504,6 +504,14 @@ struct Tracer {
 
   // Whether the surface of primitive ci perturbs the shading normal with
   // noise (the test of normal_map_noise).
+  // The shade bin (kShadeBins) of a closest-hit result: 0 for a miss, the hit
+  // surface's material for 1..5, 6 for any other material value.
+  __device__ __forceinline__ int shade_bin(int ci) const {
+    if (ci < 0) return 0;
+    const int mat = jf2i(S.surfaces[S.prims[ci].surface_offset]);
+    return mat >= 1 && mat <= 5 ? mat : 6;
+  }
+
   __device__ __forceinline__ bool has_noise_normals(int ci) const {
     if (P.normal_mapping != 1 || ci < 0) return false;
     const int so = S.prims[ci].surface_offset;
@@ -937,16 +945,22 @@ struct Tracer {
   // Returns true while the path continues with the new ray in (o, d) and
   // throughput m; false when it has ended with radiance p (pending the shadow
   // query).  depth counts the bounces taken.
+  // bin: the shade bin (kShadeBins) the path was binned in, or -1 (none).
+  // A bin fixes whether the path missed and, for bins 1..5, the material of
+  // the hit surface; a wave draining one bin then takes the same branches of
+  // the material switch on every lane.  The arithmetic of every path is the
+  // same for every bin.
   __device__ __forceinline__ bool pt_bounce(float o[3], float d[3], float m[3], float p[3], int& depth,
                                             bool sky_active, bool sun_active, bool& shadow, float sd[3],
-                                            float pv[3], Vtx* vtx = nullptr) {
+                                            float pv[3], Vtx* vtx = nullptr, int bin = -1) {
     const int depth_rr = P.depth_rr;
     float ox = o[0], oy = o[1], oz = o[2], dx = d[0], dy = d[1], dz = d[2];
     float pr = p[0], pg = p[1], pb = p[2];
     float mr = m[0], mg = m[1], mb = m[2];
     shadow = false;
     const int po = h.prim;
-    if (h.t == DF_INF || po == -1) {
+    const bool missed = bin == 0 ? true : bin > 0 ? false : (h.t == DF_INF || po == -1);
+    if (missed) {
       if (vtx) vtx->miss = true;
       sky(sky_active, dx, dy, dz);
       pr += mr * tmp[0] * DF_PI_RECIP;
@@ -995,7 +1009,7 @@ struct Tracer {
       eG = ch_g(emi);
       eB = ch_b(emi);
     }
-    const int mat = jf2i(S.surfaces[so]);
+    const int mat = bin >= 1 && bin <= 5 ? bin : jf2i(S.surfaces[so]);
     float uv[2] = {0.0f, 0.0f};
     if (mat == 1 || mat == 3 || mat == 4) {
       shadow = sun_sample(sun_active, pX, pY, pZ, wX, wY, wZ, aR, aG, aB, sd, uv);

@@ -32,6 +32,8 @@
 // indexed by pixel and lives in DPaths.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "df_kernels.h"
 #include "df_pass_common.h"
 
@@ -66,6 +68,30 @@ __device__ __forceinline__ void wave_append(int32_t* q, int64_t cap, int32_t* co
   if (lane == leader) base = atomicAdd(counts + stripe * kLineInts, __popcll(mask));
   base = __shfl(base, leader, 64);
   if (pred) q[stripe * cap + base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// Appends `value` for the lanes with `pred` set to bin `key` of the binned
+// queue q (bins of kQueueStripes stripes of `cap` entries; count groups of
+// the bins kCtlGroup ints apart): per distinct key in the wave one ballot and
+// one atomic, on the count of the wave's stripe of that bin.
+__device__ __forceinline__ void wave_append_keyed(int32_t* q, int64_t cap, int32_t* counts, bool pred, int key,
+                                                  int32_t value) {
+  unsigned long long rest = __ballot(pred);
+  const int lane = __lane_id();
+  const int stripe = global_wave() % kQueueStripes;
+  while (rest != 0ull) {
+    const int leader = __ffsll(static_cast<unsigned long long>(rest)) - 1;
+    const int k = __shfl(key, leader, 64);
+    const unsigned long long mask = __ballot(pred && key == k);
+    rest &= ~mask;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(counts + static_cast<int64_t>(k) * kCtlGroup + stripe * kLineInts,
+                                         __popcll(mask));
+    base = __shfl(base, leader, 64);
+    if (pred && key == k)
+      q[(static_cast<int64_t>(k) * kQueueStripes + stripe) * cap + base + __popcll(mask & ((1ull << lane) - 1ull))] =
+          value;
+  }
 }
 
 // Runs body(valid, entry) over the striped queue q (count group `counts`,
@@ -134,7 +160,15 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
 
 // Occupancy hint: four waves per SIMD (128 VGPRs, a few spills) measured
 // faster than the three an unconstrained allocation allows.
-template <int M>
+//
+// kBinned: the paths come in the trace kernel's shade bins (counts `count`
+// and heads `head` are then the bins' groups, kCtlGroup ints apart), drained
+// one bin after the other, so no wave mixes materials and the material switch
+// branches the same way on every lane; else they come in the bounce's live
+// queue q.  (Bodies specialised per bin at compile time measured slower: the
+// seven copies made the kernel 93k instructions against 18k, past what the
+// instruction cache holds.)
+template <int M, bool kBinned>
 __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
@@ -145,7 +179,7 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
   tr.noise_in = W.noise;  // computed by wf_normals
   const bool sky_active = K.sky_active == 1;
   const bool sun_active = K.sun_active == 1;
-  drain(q, W.n, count, head, [&](bool valid, int32_t i) {
+  auto shade = [&](int bin, bool valid, int32_t i) {
     bool more = false, shadow = false;
     if (valid) {
       tr.noise_index = i;
@@ -157,7 +191,7 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
       tr.rng.s = F.seeds[i];
       int depth = bounce;
       float sd[3], pv[3];
-      more = tr.pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv);
+      more = tr.pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv, nullptr, bin);
       F.seeds[i] = tr.rng.s;
       W.rad[i] = f4(p, 0.0f);
       if (shadow) {
@@ -175,7 +209,17 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
     }
     wave_append(q_next, W.n, count_next, valid && more, i);
     wave_append(W.shadow_queue, W.n, count_shadow, valid && shadow, more ? i : (i | kFinish));
-  });
+  };
+  if constexpr (kBinned) {
+    for (int b = 0; b < kShadeBins; ++b) {
+      // bin 6 (materials outside 1..5) reads the material at run time, as without bins
+      const int as = b < 6 ? b : -1;
+      drain(W.shade_bins + static_cast<int64_t>(b) * kQueueStripes * W.n, W.n, count + b * kCtlGroup,
+            head + b * kCtlGroup, [&](bool valid, int32_t i) { shade(as, valid, i); });
+    }
+  } else {
+    drain(q, W.n, count, head, [&](bool valid, int32_t i) { shade(-1, valid, i); });
+  }
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   take_counters(tr, cnt);
   flush_counters(cnt, F.counters, kStageShade);
@@ -229,11 +273,12 @@ template <int M, bool kEscape>
 __global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
-                                                     int first_bounce, int32_t* noise_count) {
+                                                     int first_bounce, int32_t* noise_count, int32_t* bin_count) {
   Tracer<M> tr(S, C, K, P, kPerm);
   const bool sky_active = K.sky_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     bool noisy = false;
+    int bin = 0;
     if (valid) {
       const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
       if constexpr (kEscape) {
@@ -256,9 +301,13 @@ __global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, 
         W.hit[i] = make_float4(t, __int_as_float(ci), __int_as_float(cst), __int_as_float(cso));
         if (first_bounce) F.primitive_offsets[i] = ci == -1 ? -1 : ci * 3;
         noisy = tr.has_noise_normals(ci);
+        if (P.shade_binned) bin = tr.shade_bin(ci);
       }
     }
-    if constexpr (!kEscape) wave_append(W.noise_queue, W.n, noise_count, noisy, i);
+    if constexpr (!kEscape) {
+      wave_append(W.noise_queue, W.n, noise_count, noisy, i);
+      if (P.shade_binned) wave_append_keyed(W.shade_bins, W.n, bin_count, valid, bin, i);
+    }
   });
   drain(W.shadow_queue, W.n, shadow_count, shadow_head, [&](bool valid, int32_t e) {
     if (!valid) return;
@@ -348,13 +397,14 @@ int resident_blocks(Kernel k) {
 // Persistent grid sizes of the queue kernels, queried once per math mode (before
 // any graph capture: the occupancy queries are not stream work).
 struct Grids {
-  int trace, shade, escape, normals;
+  int trace, shade, shade_binned, escape, normals;
 };
 
 template <int M>
 const Grids& grids() {
-  static const Grids g{resident_blocks(wf_trace<M, false>), resident_blocks(wf_shade<M>),
-                       resident_blocks(wf_trace<M, true>), resident_blocks(wf_normals<M>)};
+  static const Grids g{resident_blocks(wf_trace<M, false>), resident_blocks(wf_shade<M, false>),
+                       resident_blocks(wf_shade<M, true>), resident_blocks(wf_trace<M, true>),
+                       resident_blocks(wf_normals<M>)};
   return g;
 }
 
@@ -365,7 +415,8 @@ template <int M>
 hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
                                  const DPaths& w, hipStream_t stream, const dfk_marks* marks, int clear, int last) {
   const Grids& g = grids<M>();
-  const int g_trace = g.trace, g_shade = g.shade, g_escape = g.escape, g_normals = g.normals;
+  const int g_trace = g.trace, g_shade = g.shade, g_shade_binned = g.shade_binned, g_escape = g.escape,
+            g_normals = g.normals;
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
@@ -379,6 +430,9 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
   auto head = [&](int h) { return group(2 * bounces + 2 + h); };
   auto noise_count = [&](int b) { return group(5 * bounces + 6 + 2 * b); };
   auto noise_head = [&](int b) { return group(5 * bounces + 7 + 2 * b); };
+  // bounce b's shade bins: kShadeBins count groups, then kShadeBins head groups
+  auto bin_count = [&](int b) { return group(7 * bounces + 6 + 2 * kShadeBins * b); };
+  auto bin_head = [&](int b) { return group(7 * bounces + 6 + 2 * kShadeBins * b + kShadeBins); };
   const dim3 pix_block(16, 16);
   const dim3 pix_grid((p.width + 15) / 16, (p.height + 15) / 16);
   auto mark = [&](int stage, int begin) {
@@ -394,13 +448,20 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
       int32_t* sc = b > 0 ? shadow_count(b - 1) : shadow_count(bounces);
       mark(kStageTrace, 1);
       hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
-                         head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b));
+                         head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b),
+                         bin_count(b));
       mark(kStageTrace, 0);
       mark(kStageShade, 1);  // the noise normal maps count as shading
       hipLaunchKernelGGL(wf_normals<M>, dim3(g_normals), dim3(kThreads), 0, stream, s, c, k, p, w, noise_count(b),
                          noise_head(b));
-      hipLaunchKernelGGL(wf_shade<M>, dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w, w.queue[b & 1],
-                         live(b), head(3 * b + 2), w.queue[(b + 1) & 1], live(b + 1), shadow_count(b), b);
+      if (p.shade_binned)
+        hipLaunchKernelGGL((wf_shade<M, true>), dim3(g_shade_binned), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                           w.queue[b & 1], bin_count(b), bin_head(b), w.queue[(b + 1) & 1], live(b + 1),
+                           shadow_count(b), b);
+      else
+        hipLaunchKernelGGL((wf_shade<M, false>), dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+                           w.queue[b & 1], live(b), head(3 * b + 2), w.queue[(b + 1) & 1], live(b + 1),
+                           shadow_count(b), b);
       mark(kStageShade, 0);
     }
     // the last bounce's sun queries and the escape tests of the paths that
@@ -409,7 +470,7 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
     mark(kStageEscape, 1);
     hipLaunchKernelGGL((wf_trace<M, true>), dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
                        shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], live(bounces),
-                       head(3 * bounces + 1), 0, nullptr);
+                       head(3 * bounces + 1), 0, nullptr, nullptr);
     mark(kStageEscape, 0);
     mark(kStageFilm, 1);
     hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w, clear, last);

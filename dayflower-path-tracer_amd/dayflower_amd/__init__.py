@@ -16,6 +16,8 @@ from .renderer import (  # noqa: F401
     PT_AUTO,
     PT_MEGAKERNEL,
     PT_WAVEFRONT,
+    TILE_ROWS,
+    TILE_SQUARE,
     device_count,
     measure_roof,
 )

@@ -88,6 +88,11 @@ class Diff(ctypes.Structure):
     ]
 
 
+class Schedule(ctypes.Structure):
+    _fields_ = [("engine", ctypes.c_int32), ("wave_tile", ctypes.c_int32), ("shade_binned", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
 class Params(ctypes.Structure):
     _fields_ = [
         ("renderer_type", ctypes.c_int32),
@@ -180,6 +185,8 @@ HIP_PROTOTYPES = {
     "df_batch_execute_noalloc": (ctypes.c_int, [_P]),
     "df_batch_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "df_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "df_get_schedule": (ctypes.c_int, [_P, ctypes.POINTER(Schedule)]),
+    "df_set_schedule": (ctypes.c_int, [_P, ctypes.POINTER(Schedule)]),
     "df_batch_last_execute_ms": (ctypes.c_int, [_P, c_float_p]),
     "df_batch_last_error": (ctypes.c_char_p, [_P]),
 }

@@ -38,6 +38,8 @@ MATH_EXACT = 1
 PT_AUTO = 0
 PT_MEGAKERNEL = 1
 PT_WAVEFRONT = 2
+TILE_ROWS = 0
+TILE_SQUARE = 1
 
 
 class HIPError(RuntimeError):
@@ -198,6 +200,21 @@ class HIPRendererKernel:
     def execute(self, passes=1):
         """execute(range) + clearFilmFlags(), ``passes`` times (asynchronous)."""
         self._check(self._lib.df_execute(self._ctx, int(passes)), "df_execute")
+
+    def schedule(self):
+        """df_get_schedule: {"engine", "wave_tile", "shade_binned"} of the next passes (engine PT_AUTO while the
+        autotuner is still timing its candidates)."""
+        sc = _ffi.Schedule()
+        self._check(self._lib.df_get_schedule(self._ctx, ctypes.byref(sc)), "df_get_schedule")
+        return {"engine": sc.engine, "wave_tile": sc.wave_tile, "shade_binned": sc.shade_binned}
+
+    def setSchedule(self, engine=None, wave_tile=0, shade_binned=0):
+        """df_set_schedule: fix the path tracer's schedule (engine None: back to the parameters' pt_engine)."""
+        if engine is None:
+            self._check(self._lib.df_set_schedule(self._ctx, None), "df_set_schedule")
+        else:
+            sc = _ffi.Schedule(int(engine), int(wave_tile), int(shade_binned), 0)
+            self._check(self._lib.df_set_schedule(self._ctx, ctypes.byref(sc)), "df_set_schedule")
 
     def setGraphs(self, enabled):
         """df_set_graphs: replay the frame loop's launches from captured HIP graphs (default off)."""

@@ -104,10 +104,19 @@ int df_clear(df_ctx* ctx);
  * pixel per pass.  Asynchronous on the handle's stream. */
 int df_execute(df_ctx* ctx, int32_t passes);
 
-/* The path tracer's schedule for the next df_execute: DF_PT_MEGAKERNEL or
+/* The path tracer's engine for the next df_execute: DF_PT_MEGAKERNEL or
  * DF_PT_WAVEFRONT, or DF_PT_AUTO while the autotuner is still timing the
- * two on the current frame (no counterpart in the reference). */
+ * schedules on the current frame (no counterpart in the reference). */
 int df_get_engine(df_ctx* ctx, int32_t* engine);
+
+/* The full schedule of the next passes (engine DF_PT_AUTO while the
+ * autotuner is still timing its candidates), and a fixed schedule for every
+ * later df_execute (NULL: back to params.pt_engine, re-tuning DF_PT_AUTO).
+ * No counterpart in the reference; every schedule gives the same bits.  In a
+ * batch, a path-tracer job whose schedule is the wavefront is rendered on it
+ * by df_batch_execute instead of by the batch kernel. */
+int df_get_schedule(df_ctx* ctx, df_schedule* schedule);
+int df_set_schedule(df_ctx* ctx, const df_schedule* schedule);
 
 /* Frame-loop graphs (no counterpart in the reference): with graphs on
  * (default off: on an MI355X replaying measured 1-7 % slower per pass than
@@ -204,7 +213,8 @@ int df_batch_execute(df_batch* batch);
  * array of about 4.7 MiB per GPU, set up by df_batch_create and held until
  * the process ends.  Limits: at most 1024 scenes and 262144 16x16 tiles
  * (their sum over the scenes) per call, else DF_EINVAL with a message and
- * nothing rendered.  Results equal df_batch_execute's bit for bit.  Calls
+ * nothing rendered.  Results equal df_batch_execute's bit for bit (every job
+ * through the batch kernel, whatever its own schedule).  Calls
  * share their GPU's array: this process's calls are serialised per device
  * and each returns after its batch is done; calls that overlapped on one GPU
  * would race. */

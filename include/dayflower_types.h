@@ -59,13 +59,30 @@ enum { DF_SHADER_FLAT = 1, DF_SHADER_GOURAUD = 2 };
 enum { DF_MATH_FAST = 0, DF_MATH_EXACT = 1 };
 
 /* How the path tracer is scheduled on the GPU; every choice gives the same
- * bits.  AUTO times both schedules on the first passes of a frame (one pass
- * at a time, alternating, two passes each) and keeps the faster one until
- * the scene, resolution or parameters change.
+ * bits.  AUTO times the candidate schedules (df_schedule: the megakernel with
+ * strip or square waves, the wavefront with one shade queue or material
+ * bins) on the first passes of a frame, one pass at a time, two passes each,
+ * and keeps the fastest until the scene, resolution or parameters change.
  * DF_PT_MEGAKERNEL : one work-item runs a pixel's whole path.
  * DF_PT_WAVEFRONT  : a kernel per stage (closest-hit, shade, shadow) over
  *                    queues of live paths. */
 enum { DF_PT_AUTO = 0, DF_PT_MEGAKERNEL = 1, DF_PT_WAVEFRONT = 2 };
+
+/* A launch schedule of the path tracer (no counterpart in the reference):
+ * what the autotuner chooses among for DF_PT_AUTO (df_get_schedule), or what
+ * df_set_schedule fixes.  Every schedule gives the same bits.
+ *   engine       DF_PT_MEGAKERNEL or DF_PT_WAVEFRONT;
+ *   wave_tile    megakernel: the pixels of a wave within its 16x16 tile,
+ *                DF_TILE_ROWS (a 16x4 strip) or DF_TILE_SQUARE (8x8);
+ *   shade_binned wavefront: 1 = hits are binned by material and each bin
+ *                is shaded by code for that material only, 0 = one queue. */
+enum { DF_TILE_ROWS = 0, DF_TILE_SQUARE = 1 };
+typedef struct df_schedule {
+  int32_t engine;
+  int32_t wave_tile;
+  int32_t shade_binned;
+  int32_t reserved;
+} df_schedule;
 
 /* Kernel stages, for per-stage counters and timings (df_get_stage_stats).
  * MEGAKERNEL covers every renderer run as one kernel; the others are the

@@ -358,8 +358,8 @@ def test_error_paths(scenes, gpu):
 
 @pytest.mark.parametrize("cid,w,h", [(2, 128, 96), (3, 160, 90)])
 def test_autotuned_schedule_gives_the_same_bits(scenes, gpu, cid, w, h):
-    """PT_AUTO alternates the megakernel and the wavefront schedule one pass at a time while it times them, then
-    keeps one: the film equals a megakernel-only render of the same passes."""
+    """PT_AUTO times its candidate schedules one pass at a time (four candidates, two passes each), then keeps
+    one: the film equals a megakernel-only render of the same passes."""
     sc = _small(scenes(cid), w, h)
     out = []
     for engine in (dfa.PT_AUTO, dfa.PT_MEGAKERNEL):
@@ -371,8 +371,8 @@ def test_autotuned_schedule_gives_the_same_bits(scenes, gpu, cid, w, h):
             k.setParams(p)
             k.update(w, h)
             k.seed(11)
-            k.execute(3)  # tuning: one pass at a time
-            k.execute(4)
+            k.execute(5)  # tuning: one pass at a time
+            k.execute(4)  # the last three tuning passes, then the kept schedule
             out.append((k.getFilm(), k.getSeeds(), k.engine()))
         finally:
             k.dispose()
