@@ -360,3 +360,105 @@ def test_material_showcase_scene_compiles():
     assert tex_types.count(6) == 2
     # eye (295, 42, 332.6) with the default orientation
     np.testing.assert_array_equal(sc.camera[1:4], np.array([295.0, 42.0, 332.6], np.float32))
+
+
+# ---------------------------------------------------------------- the reference's own asset files
+# tests/golden/Texture_2.png, bricks2_normal.jpg, monkey0.obj and smoothMonkey0.obj are data files the reference
+# ships (resources/distribution/resources/texture, .../model); Scenes.java:356,376 load the two images.
+
+def _png_rgba(path):
+    """A minimal independent PNG decoder (8-bit, non-interlaced; RGB, RGBA or palette with tRNS): the filtered
+    scanlines of the concatenated IDAT stream unfiltered per the PNG specification, section 9."""
+    import struct
+    import zlib
+
+    data = open(path, "rb").read()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    i, chunks, idat = 8, {}, b""
+    while i < len(data):
+        n, = struct.unpack(">I", data[i:i + 4])
+        kind, body = data[i + 4:i + 8], data[i + 8:i + 8 + n]
+        if kind == b"IDAT":
+            idat += body
+        else:
+            chunks[kind] = body
+        i += 12 + n
+    w, h, depth, ctype, _, _, interlace = struct.unpack(">IIBBBBB", chunks[b"IHDR"])
+    assert depth == 8 and interlace == 0 and ctype in (2, 3, 6)
+    bpp = {2: 3, 3: 1, 6: 4}[ctype]
+    raw = np.frombuffer(zlib.decompress(idat), np.uint8).reshape(h, 1 + w * bpp)
+    out = np.zeros((h, w * bpp), np.int64)
+    for y in range(h):
+        f, line = raw[y, 0], raw[y, 1:].astype(np.int64)
+        prev = out[y - 1] if y else np.zeros(w * bpp, np.int64)
+        cur = np.zeros(w * bpp, np.int64)
+        for x in range(w * bpp):
+            a = cur[x - bpp] if x >= bpp else 0
+            b = prev[x]
+            c = prev[x - bpp] if x >= bpp else 0
+            if f == 0:
+                pred = 0
+            elif f == 1:
+                pred = a
+            elif f == 2:
+                pred = b
+            elif f == 3:
+                pred = (a + b) // 2
+            else:
+                p = a + b - c
+                pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+                pred = a if pa <= pb and pa <= pc else b if pb <= pc else c
+            cur[x] = (line[x] + pred) & 0xFF
+        out[y] = cur
+    if ctype == 3:
+        pal = np.frombuffer(chunks[b"PLTE"], np.uint8).reshape(-1, 3).astype(np.int64)
+        alpha = np.full(len(pal), 255, np.int64)
+        trns = np.frombuffer(chunks.get(b"tRNS", b""), np.uint8)
+        alpha[:len(trns)] = trns
+        return w, h, pal[out], alpha[out]
+    px = out.reshape(h, w, bpp)
+    return w, h, px[:, :, :3], px[:, :, 3] if bpp == 4 else np.full((h, w), 255, np.int64)
+
+
+def test_reference_png_texture_decodes_exactly():
+    """Texture_2.png (an 8-bit palette image whose tRNS chunk makes palette entry 0 transparent; no pixel uses
+    that entry) as ImageTexture.load keeps it: the TYPE_INT_RGB raster of the image drawn over black, i.e.
+    opaque texels as stored, a texel of alpha a scaled by a / 255 rounded to nearest."""
+    path = os.path.join(GOLDEN, "Texture_2.png")
+    w, h, rgb, alpha = _png_rgba(path)
+    assert (w, h) == (512, 512)
+    over_black = np.where(alpha[..., None] == 255, rgb, (alpha[..., None] * rgb + 127) // 255)
+    want = (over_black[..., 0] << 16) | (over_black[..., 1] << 8) | over_black[..., 2]
+    gw, gh, px = assets.image_rgb(path)
+    assert (gw, gh) == (w, h)
+    np.testing.assert_array_equal(px, want.reshape(-1).astype(np.int32))
+    assert np.all(alpha == 255) and len(np.unique(px)) > 8
+
+
+def test_reference_jpeg_normal_map_decodes():
+    """bricks2_normal.jpg: 512 x 512 baseline JPEG; a tangent-space normal map, so its blue channel dominates.
+    (The decode's agreement with ImageIO's is unpinned: no JVM here.)"""
+    w, h, px = assets.image_rgb(os.path.join(GOLDEN, "bricks2_normal.jpg"))
+    assert (w, h) == (512, 512)
+    r, g, b = (px >> 16) & 0xFF, (px >> 8) & 0xFF, px & 0xFF
+    assert b.mean() > 200 and abs(r.mean() - 128) < 20 and abs(g.mean() - 128) < 20
+
+
+@pytest.mark.parametrize("name", ["monkey0.obj", "smoothMonkey0.obj"])
+def test_reference_obj_models_load(name):
+    """Blender's Suzanne as the reference ships it, flat (monkey0) and smooth (smoothMonkey0): every face fanned
+    from its first vertex in file order, positions and given normals kept as written, v flipped to 1 - v."""
+    path = os.path.join(GOLDEN, name)
+    n, sc = _load(path)
+    v, vt, vn, faces = _obj_tokens(path)
+    fan = [(f[0], f[k], f[k + 1]) for f in faces for k in range(1, len(f) - 1)]
+    assert n == 1 and sc.primitive_count == 1
+    pos, nrm, uv, _ = _tris(sc)
+    assert pos.shape[0] == len(fan) == sum(len(f) - 2 for f in faces)
+    for i, tri in enumerate(fan):
+        for k, (vi, ti, ni) in enumerate(tri):
+            np.testing.assert_array_equal(pos[i, k], v[vi])
+            np.testing.assert_array_equal(nrm[i, k], vn[ni])
+            assert uv[i, k, 1] == np.float32(np.float32(1.0) - vt[ti, 1])
+    stats = dfa.scene.stats(sc)
+    assert stats["triangles"] == len(fan)

@@ -3,9 +3,12 @@
 Bars (exact math, every renderer mode, bit for bit against the oracle: film, samples, BGRA, float image,
 primitive offsets, seeds):
 * the material showcase scene of Scenes.java:318-408 (plane + 15 spheres placed by transforms, every texture
-  kind, an image albedo and an image normal map) with synthetic images of the reference's kind;
+  kind, an image albedo and an image normal map) with the two images it loads, the reference's own
+  Texture_2.png and bricks2_normal.jpg (tests/golden);
 * the icosphere OBJ model the reference's own BVH test loads, under a rotation + non-uniform scale + translation,
-  with an image albedo and an image normal map, over a checkerboard plane.
+  with an image albedo and an image normal map (synthetic images), over a checkerboard plane;
+* the reference's smoothMonkey0.obj (Suzanne with its own smooth normals) with Texture_2.png as albedo and
+  bricks2_normal.jpg as normal map.
 """
 
 import os
@@ -40,14 +43,30 @@ def _obj_scene():
         return b.compile()
 
 
+def _monkey_scene():
+    with dfa.SceneBuilder("Monkey_Scene") as b:
+        black = b.constant((0.0, 0.0, 0.0))
+        s = b.surface(assets.LAMBERTIAN, b.image(os.path.join(GOLDEN, "Texture_2.png"), 0.0, 1.0, 1.0), black,
+                      b.image(os.path.join(GOLDEN, "bricks2_normal.jpg"), 0.0, 2.0, 2.0))
+        b.load_obj(os.path.join(GOLDEN, "smoothMonkey0.obj"), s,
+                   assets.transform((0.0, 1.1, 0.0), (0.0, 1.0, 0.0, 0.6), (1.0, 1.0, 1.0)))
+        ground = b.surface(assets.LAMBERTIAN, b.checkerboard((0.3, 0.3, 0.3), (0.9, 0.9, 0.9), 0.5, 0.5), black,
+                           black)
+        b.plane((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (1.0, 0.0, 0.0), ground)
+        b.set_camera(dfa.camera_setup(W, H, 45.0, (1.5, 2.2, 3.5), (0.0, 1.0, 0.0)))
+        b.set_sky(dfa.sky_at(35.0, 60.0, 2.5))
+        return b.compile()
+
+
 @pytest.fixture(scope="module")
 def asset_scenes():
-    t2, nrm = _synthetic_textures()
-    return {"showcase": assets.material_showcase_scene(t2, nrm, width=W, height=H), "obj": _obj_scene()}
+    t2, nrm = os.path.join(GOLDEN, "Texture_2.png"), os.path.join(GOLDEN, "bricks2_normal.jpg")
+    return {"showcase": assets.material_showcase_scene(t2, nrm, width=W, height=H), "obj": _obj_scene(),
+            "monkey": _monkey_scene()}
 
 
 @pytest.mark.parametrize("renderer", [1, 2, 3, 4, 5, 6])
-@pytest.mark.parametrize("name", ["showcase", "obj"])
+@pytest.mark.parametrize("name", ["showcase", "obj", "monkey"])
 def test_asset_scenes_render_bit_exact(gpu, asset_scenes, name, renderer):
     sc = asset_scenes[name]
     p = sc.copy_params()
