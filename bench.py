@@ -211,31 +211,36 @@ def traffic_from_profiles(workload, kernel_symbol):
 def adjoint_measure():
     """The differentiation workload (512x512, 32 samples per pixel, depth 6, 347,680 triangles): the forward
     render and the path-replay adjoint of L = sum (film - target)^2 with respect to 64 RGB albedo groups, 4
-    emissions, turbidity and sun elevation (target: a render at the hidden parameters of seed 500; current
-    parameters from seed 501), device time of each, after the schedule autotuner has settled."""
+    emissions, the checkerboard ground's two RGB colours, turbidity, sun elevation and sun azimuth (target: a
+    render at the hidden parameters of seed 500; current parameters from seed 501), device time of each, after
+    the schedule autotuner has settled."""
     import numpy as np
 
     import dayflower_amd as dfa
 
     sc = dfa.build_config(5)
     sa, se, na, ne = dfa.diff_maps(sc)
+    sk, nc, _ = dfa.diff_checker_map(sc)
 
     def draw(seed):
         r = np.random.default_rng(seed)
         return (r.uniform(0.1, 0.9, (na, 3)).astype(np.float32), r.uniform(1.0, 20.0, ne).astype(np.float32),
-                float(r.uniform(2.0, 6.0)), float(r.uniform(10.0, 70.0)))
+                r.uniform(0.1, 0.9, (nc, 2, 3)).astype(np.float32), float(r.uniform(2.0, 6.0)),
+                float(r.uniform(10.0, 70.0)), float(r.uniform(0.0, 90.0)))
+
+    def set_diff(k, th):
+        a, e, c, t, el, az = th
+        k.setDifferentiable(sa, se, a, e, t, el, az, surface_checker=sk, checker=c)
 
     k = dfa.HIPRendererKernel(sc)
     try:
         k.setParams(sc.copy_params())  # depth 6, Russian roulette off
         k.update(sc.width, sc.height)
-        a, e, t, el = draw(500)
-        k.setDifferentiable(sa, se, a, e, t, el, 30.0)
+        set_diff(k, draw(500))
         k.seed(5)
         k.lossGradient(np.zeros(sc.width * sc.height * 3, np.float32), sc.spp, gradient=False)
         target = k.getFilm()[0].copy()
-        a, e, t, el = draw(501)
-        k.setDifferentiable(sa, se, a, e, t, el, 30.0)
+        set_diff(k, draw(501))
         runs = []
         for _ in range(3):
             k.seed(7)

@@ -152,24 +152,48 @@ struct DParams {
   // sRGB constants (image4j RGBColorSpace.SRGB)
   float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
   // differentiable mode (diff_on = 1): continuous albedo / emission
-  // parameters replace the 8-bit textures of the surfaces mapped to them
+  // parameters replace the 8-bit textures of the surfaces mapped to them.
+  // Albedo slots: k < checker_base is albedo group k; checkerboard group g
+  // has slots checker_base + 2 g + t, t the tile (0: the texture's first
+  // colour); diff_albedo holds every slot's RGB.
   int32_t diff_on;
   int32_t n_surfaces;               // surface records (6 floats each)
   const int32_t* surface_albedo;    // per surface record: albedo group or -1
   const int32_t* surface_emission;  // per surface record: emission index or -1
-  const float* diff_albedo;         // groups x RGB
+  const int32_t* surface_checker;   // per surface record: checkerboard group or -1
+  const float* diff_albedo;         // albedo slots x RGB
   const float* diff_emission;       // white radiance per index
+  int32_t checker_base;
+};
+
+// A double with its derivatives with respect to the sky's parameters:
+// turbidity, sun elevation and sun azimuth (degrees).
+struct Dual3 {
+  double v;
+  double d[3];
+};
+
+// Sky.set of the sky's parameters in dual numbers: what the adjoint
+// differentiates the sky radiance and the sun sample through.
+struct DSkyDual {
+  float onb_u[3], onb_v[3], onb_w[3];
+  Dual3 world[3];  // unit vector towards the sun, world frame
+  Dual3 dir[3];    // the same in the sky frame
+  Dual3 theta;     // the sun's zenith angle
+  Dual3 zl, zx, zy;
+  Dual3 pl[5], px[5], py[5];
 };
 
 // Inputs of the adjoint (path-replay) kernel beyond those of a render.
+// Gradient layout: albedo groups x RGB, emissions, checkerboard groups x 2
+// colours x RGB, turbidity, sun elevation, sun azimuth (per degree).
 struct DAdjoint {
-  // device array of 4 skies: turbidity + h_t, - h_t, sun elevation + h_e, - h_e
-  const DSky* sky_off;
-  float h_t, h_e;       // the offsets (turbidity units, degrees)
+  const DSkyDual* sky;  // device copy of the dual Sky.set
   const float* weight;  // dLoss/dfilm per pixel and channel, divided by the passes
-  double* grad;         // albedo groups x 3, emission, turbidity, elevation
+  double* grad;
   int32_t n_albedo;
   int32_t n_emission;
+  int32_t n_checker;
 };
 
 // Parameters of the adjoint that one launch handles: LDS holds them per block.

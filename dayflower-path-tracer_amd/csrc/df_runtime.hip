@@ -19,6 +19,7 @@
 
 #include "../../include/dayflower_hip.h"
 #include "df_device_scene.h"
+#include "df_dual.h"
 #include "df_kernels.h"
 
 using dfk::DBatchItem;
@@ -116,10 +117,11 @@ struct df_ctx {
 
   // differentiable mode
   bool diff_on = false;
-  int32_t diff_n_albedo = 0, diff_n_emission = 0;
-  float diff_h_t = 0.0f, diff_h_e = 0.0f;
-  DevBuf diff_surf_albedo, diff_surf_emission, diff_albedo, diff_emission, diff_skies, diff_weight, diff_grad,
-      diff_loss, diff_seeds;
+  int32_t diff_n_albedo = 0, diff_n_emission = 0, diff_n_checker = 0;
+  double diff_sky_params[3] = {0.0, 0.0, 0.0};  // turbidity, sun elevation, sun azimuth
+  std::vector<int32_t> diff_checker_host;       // per surface record: checkerboard group or -1
+  DevBuf diff_surf_albedo, diff_surf_emission, diff_surf_checker, diff_albedo, diff_emission, diff_sky_dual,
+      diff_weight, diff_grad, diff_loss, diff_seeds;
   size_t diff_n_surfaces = 0;
 
   // per-stage launch timing: event pairs recorded around each launch
@@ -229,6 +231,12 @@ inline int32_t jf2i(float v) {
   if (v >= 2147483648.0f) return INT32_MAX;
   if (v <= -2147483648.0f) return INT32_MIN;
   return static_cast<int32_t>(v);
+}
+
+// Whether surface record i's albedo texture is a checkerboard texture.
+bool checker_albedo(const std::vector<float>& tex, const float* surfaces, size_t i) {
+  const int32_t off = jf2i(surfaces[i * 6 + 1]);
+  return off >= 0 && static_cast<size_t>(off) + 8 <= tex.size() && jf2i(tex[off]) == 3;
 }
 
 inline bool in_range(int64_t off, int64_t count, size_t len) { return off >= 0 && off + count <= static_cast<int64_t>(len); }
@@ -884,8 +892,10 @@ void apply_diff(const df_ctx* c, DParams& p) {
   p.n_surfaces = static_cast<int32_t>(c->diff_n_surfaces);
   p.surface_albedo = static_cast<const int32_t*>(c->diff_surf_albedo.p);
   p.surface_emission = static_cast<const int32_t*>(c->diff_surf_emission.p);
+  p.surface_checker = static_cast<const int32_t*>(c->diff_surf_checker.p);
   p.diff_albedo = static_cast<const float*>(c->diff_albedo.p);
   p.diff_emission = static_cast<const float*>(c->diff_emission.p);
+  p.checker_base = c->diff_n_albedo;
 }
 
 DFrame frame_of(df_ctx* c) {
@@ -1192,8 +1202,9 @@ void df_destroy(df_ctx* c) {
   free_buf(c->seeds);
   free_buf(c->counters);
   free_paths(c);
-  for (DevBuf* b : {&c->diff_surf_albedo, &c->diff_surf_emission, &c->diff_albedo, &c->diff_emission, &c->diff_skies,
-                    &c->diff_weight, &c->diff_grad, &c->diff_loss, &c->diff_seeds})
+  for (DevBuf* b : {&c->diff_surf_albedo, &c->diff_surf_emission, &c->diff_surf_checker, &c->diff_albedo,
+                    &c->diff_emission, &c->diff_sky_dual, &c->diff_weight, &c->diff_grad, &c->diff_loss,
+                    &c->diff_seeds})
     free_buf(*b);
   for (auto& row : c->g_wave)
     for (GraphCache& g : row) g.reset();
@@ -1236,6 +1247,7 @@ int df_load_scene(df_ctx* c, const df_compiled_scene* s) {
   }
   std::memcpy(c->camera, s->camera, sizeof c->camera);
   c->has_scene = true;
+  c->diff_on = false;  // the parameter maps index the previous scene's surface records
   reset_tuning(c);
   return DF_OK;
 }
@@ -1314,6 +1326,11 @@ int df_upload_transforms(df_ctx* c, const float* o2w, const float* w2o, int32_t 
 int df_upload_surfaces(df_ctx* c, const float* surfaces, size_t n) {
   if (!c || !surfaces) return DF_EINVAL;
   if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
+  if (c->diff_on) {  // a checkerboard group's surface must keep a checkerboard albedo texture
+    for (size_t i = 0; i < c->diff_checker_host.size() && i * 6 + 1 < n; ++i)
+      if (c->diff_checker_host[i] >= 0 && !checker_albedo(c->hs.textures, surfaces, i))
+        return fail(c, DF_ESTATE, "surface %zu is in a checkerboard group: its albedo texture must stay one", i);
+  }
   DF_HIP(c, hipSetDevice(c->device));
   DF_HIP(c, hipStreamSynchronize(c->stream));
   std::vector<float> old = c->hs.surfaces;
@@ -1665,41 +1682,77 @@ const char* df_last_error(df_ctx* c) {
 
 // ------------------------------------------------------------ differentiable path tracer
 
+// Whether the sky the renderer holds is Sky.set of (turbidity, elevation,
+// azimuth): its sun direction within 1e-5 of the parameters' and its
+// turbidity within 1e-5 relative.
+bool sky_matches(const df_sky& k, const double q[3]) {
+  const double deg = M_PI / 180.0;
+  const double e = q[1] * deg, a = q[2] * deg;
+  const double w[3] = {std::cos(e) * std::sin(a), std::sin(e), std::cos(e) * std::cos(a)};
+  for (int i = 0; i < 3; ++i)
+    if (!(std::fabs(w[i] - k.sun_direction_world[i]) <= 1e-5)) return false;
+  return std::fabs(q[0] - k.turbidity) <= 1e-5 * std::fabs(q[0]);
+}
+
 int df_set_diff(df_ctx* c, const df_diff* d) {
   if (!c || !d) return DF_EINVAL;
   if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
   const size_t n_surf = c->hs.surfaces.size() / 6;
   if (d->n_surfaces != n_surf || !d->surface_albedo || !d->surface_emission)
     return fail(c, DF_EINVAL, "surface maps must hold one entry per surface record (%zu)", n_surf);
-  if (d->n_albedo < 0 || d->n_emission < 0 || 3 * d->n_albedo + d->n_emission + 2 > dfk::kMaxDiffParams)
-    return fail(c, DF_EINVAL, "too many parameters (%d albedo groups, %d emissions)", d->n_albedo, d->n_emission);
-  if ((d->n_albedo > 0 && !d->albedo) || (d->n_emission > 0 && !d->emission))
+  const int32_t nc = d->n_checker;
+  if (d->n_albedo < 0 || d->n_emission < 0 || nc < 0 ||
+      3 * d->n_albedo + d->n_emission + 6 * nc + 3 > dfk::kMaxDiffParams)
+    return fail(c, DF_EINVAL, "too many parameters (%d albedo groups, %d emissions, %d checkerboard groups)",
+                d->n_albedo, d->n_emission, nc);
+  if ((d->n_albedo > 0 && !d->albedo) || (d->n_emission > 0 && !d->emission) ||
+      (nc > 0 && (!d->checker || !d->surface_checker)))
     return fail(c, DF_EINVAL, "parameter arrays missing");
+  const std::vector<float>& tex = c->hs.textures;
+  const std::vector<float>& surf = c->hs.surfaces;
   for (size_t i = 0; i < n_surf; ++i) {
     if (d->surface_albedo[i] < -1 || d->surface_albedo[i] >= d->n_albedo)
       return fail(c, DF_EINVAL, "surface %zu: albedo group %d", i, d->surface_albedo[i]);
     if (d->surface_emission[i] < -1 || d->surface_emission[i] >= d->n_emission)
       return fail(c, DF_EINVAL, "surface %zu: emission index %d", i, d->surface_emission[i]);
+    const int32_t g = nc > 0 ? d->surface_checker[i] : -1;
+    if (g < -1 || g >= nc) return fail(c, DF_EINVAL, "surface %zu: checkerboard group %d", i, g);
+    if (g >= 0) {
+      if (d->surface_albedo[i] >= 0)
+        return fail(c, DF_EINVAL, "surface %zu: both an albedo group and a checkerboard group", i);
+      if (!checker_albedo(tex, surf.data(), i))
+        return fail(c, DF_EINVAL, "surface %zu: a checkerboard group needs a checkerboard albedo texture", i);
+    }
   }
-  if (!(d->h_turbidity > 0.0f) || !(d->h_elevation > 0.0f)) return fail(c, DF_EINVAL, "parameter offsets must be > 0");
+  const double q[3] = {d->turbidity, d->elevation_deg, d->azimuth_deg};
+  if (!sky_matches(c->sky, q))
+    return fail(c, DF_EINVAL, "the current sky is not Sky.set of turbidity %g, elevation %g, azimuth %g", q[0], q[1],
+                q[2]);
   DF_HIP(c, hipSetDevice(c->device));
   DF_HIP(c, hipStreamSynchronize(c->stream));
   const std::vector<int32_t> sa(d->surface_albedo, d->surface_albedo + n_surf);
   const std::vector<int32_t> se(d->surface_emission, d->surface_emission + n_surf);
-  const std::vector<float> al(d->albedo, d->albedo + 3 * d->n_albedo);
-  const std::vector<float> em(d->emission, d->emission + d->n_emission);
-  const std::vector<DSky> skies = {make_sky(d->sky_turbidity[0]), make_sky(d->sky_turbidity[1]),
-                                   make_sky(d->sky_elevation[0]), make_sky(d->sky_elevation[1])};
+  std::vector<int32_t> sc(n_surf, -1);
+  if (nc > 0) sc.assign(d->surface_checker, d->surface_checker + n_surf);
+  // albedo slots: the groups, then each checkerboard group's two colours
+  std::vector<float> al(d->albedo, d->albedo + 3 * d->n_albedo);
+  if (nc > 0) al.insert(al.end(), d->checker, d->checker + 6 * nc);
+  if (al.empty()) al.assign(3, 0.0f);
+  std::vector<float> em(d->emission, d->emission + d->n_emission);
+  if (em.empty()) em.assign(1, 0.0f);
+  const std::vector<dfk::DSkyDual> dual = {
+      dfk::sky_set_dual(c->sky.onb_u, c->sky.onb_v, c->sky.onb_w, q[0], q[1], q[2])};
   int rc;
   if ((rc = upload(c, c->diff_surf_albedo, sa)) || (rc = upload(c, c->diff_surf_emission, se)) ||
-      (rc = upload(c, c->diff_albedo, al)) || (rc = upload(c, c->diff_emission, em)) ||
-      (rc = upload(c, c->diff_skies, skies)))
+      (rc = upload(c, c->diff_surf_checker, sc)) || (rc = upload(c, c->diff_albedo, al)) ||
+      (rc = upload(c, c->diff_emission, em)) || (rc = upload(c, c->diff_sky_dual, dual)))
     return rc;
   c->diff_n_surfaces = n_surf;
   c->diff_n_albedo = d->n_albedo;
   c->diff_n_emission = d->n_emission;
-  c->diff_h_t = d->h_turbidity;
-  c->diff_h_e = d->h_elevation;
+  c->diff_n_checker = nc;
+  c->diff_checker_host = sc;
+  std::copy(q, q + 3, c->diff_sky_params);
   c->diff_on = true;
   return DF_OK;
 }
@@ -1720,8 +1773,10 @@ int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, d
   const df_params& p = c->params;
   if (p.renderer_type != DF_RENDERER_PATH_TRACER) return fail(c, DF_EINVAL, "the adjoint needs the path tracer");
   if (grad) {
-    const size_t need = static_cast<size_t>(3 * c->diff_n_albedo + c->diff_n_emission + 2);
+    const size_t need = static_cast<size_t>(3 * c->diff_n_albedo + c->diff_n_emission + 6 * c->diff_n_checker + 3);
     if (n_grad != need) return fail(c, DF_EINVAL, "gradient length %zu != %zu", n_grad, need);
+    if (!sky_matches(c->sky, c->diff_sky_params))
+      return fail(c, DF_ESTATE, "the sky changed after df_set_diff: its parameters are no longer the diff's");
     if (p.pt_depth_maximum > dfk::kMaxAdjointDepth)
       return fail(c, DF_EINVAL, "the adjoint supports pt_depth_maximum <= %d", dfk::kMaxAdjointDepth);
     if (p.pt_depth_russian_roulette <= p.pt_depth_maximum)
@@ -1759,13 +1814,12 @@ int df_loss_gradient(df_ctx* c, const float* target, size_t n, int32_t passes, d
     // replay from the forward's starting seeds; the replay leaves the seeds as the forward did
     DF_HIP(c, hipMemcpyAsync(c->seeds.p, c->diff_seeds.p, np * sizeof(uint64_t), hipMemcpyDeviceToDevice, c->stream));
     dfk::DAdjoint a{};
-    a.sky_off = static_cast<const DSky*>(c->diff_skies.p);
-    a.h_t = c->diff_h_t;
-    a.h_e = c->diff_h_e;
+    a.sky = static_cast<const dfk::DSkyDual*>(c->diff_sky_dual.p);
     a.weight = static_cast<const float*>(c->diff_weight.p);
     a.grad = static_cast<double*>(c->diff_grad.p);
     a.n_albedo = c->diff_n_albedo;
     a.n_emission = c->diff_n_emission;
+    a.n_checker = c->diff_n_checker;
     const DCamera cam = make_camera(c->camera);
     const DSky sky = make_sky(c->sky);
     DParams par = make_params(c->params, c->width, c->height, passes, false);

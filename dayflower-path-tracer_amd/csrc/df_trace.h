@@ -9,16 +9,17 @@
 #pragma once
 
 #include "df_device.h"
+#include "df_dual.h"
 
 namespace dfk {
 
 // What one path-tracer bounce contributed, for the adjoint's replay: the
-// albedo / emission parameters it used, the sun term's albedo factor and its
-// elevation derivative, and how the throughput changed (m *= s, times the
-// albedo when uses_albedo).  Inputs: the offset skies for the derivative.
+// albedo slot and emission parameter it used, the sun term's albedo factor
+// and its derivatives with respect to the sun's elevation and azimuth, and
+// how the throughput changed (m *= s, times the albedo when uses_albedo).
+// Input: the dual Sky.set the derivatives are taken through.
 struct Vtx {
-  const DSky* sky_e[2];  // in: sun elevation + h_e, - h_e
-  float h_e;             // in
+  const DSkyDual* sky;   // in
   bool miss;
   bool uses_albedo;
   bool rr;               // Russian roulette scaled the albedo
@@ -26,7 +27,8 @@ struct Vtx {
   float a[3];
   float e[3];            // emission added at the vertex
   float s;
-  float sun_w, dsun_w;
+  float sun_w;
+  float dsun_w[2];       // d sun_w / d elevation, d azimuth (per degree)
 };
 
 // The mesh walk tests leaves' triangles once this fraction of the lanes
@@ -652,11 +654,7 @@ struct Tracer {
       case 3: {  // Checkerboard :1714-1767
         const int c0 = jf2i(T[off + 2]);
         const int c1 = jf2i(T[off + 3]);
-        const float u = h.uv[0], v = h.uv[1];
-        const float ca = T[off + 4], sa = T[off + 5];
-        const float tu = modulo((u * ca - v * sa) * T[off + 6]);
-        const float tv = modulo((v * ca + u * sa) * T[off + 7]);
-        const bool dark = (tu > 0.5f) != (tv > 0.5f);
+        const bool dark = checker_tile(off) == 0;
         const float m = dark ? 0.8f : 1.2f;
         float cR = dark ? ch_r(c0) : ch_r(c1);
         float cG = dark ? ch_g(c0) : ch_g(c1);
@@ -686,6 +684,17 @@ struct Tracer {
       default:
         return 0;
     }
+  }
+
+  // The tile of the checkerboard texture at `off` under the hit's uv
+  // (:1741-1747): 0 where the texture takes its first colour, 1 the second.
+  __device__ __forceinline__ int checker_tile(int off) {
+    const float* T = S.textures;
+    const float u = h.uv[0], v = h.uv[1];
+    const float ca = T[off + 4], sa = T[off + 5];
+    const float tu = modulo((u * ca - v * sa) * T[off + 6]);
+    const float tv = modulo((v * ca + u * sa) * T[off + 7]);
+    return (tu > 0.5f) != (tv > 0.5f) ? 0 : 1;
   }
 
   __device__ __forceinline__ void image_coords(int off, float* u2, float* v2, int* x, int* y, int* w, int* res) {
@@ -986,6 +995,8 @@ struct Tracer {
       if (rec_i >= 0 && rec_i < P.n_surfaces) {
         aslot = P.surface_albedo[rec_i];
         eslot = P.surface_emission[rec_i];
+        const int grp = aslot < 0 ? P.surface_checker[rec_i] : -1;
+        if (grp >= 0) aslot = P.checker_base + 2 * grp + checker_tile(jf2i(S.surfaces[so + 1]));
       }
     }
     if (aslot >= 0) {
@@ -1027,13 +1038,14 @@ struct Tracer {
       vtx->e[1] = eG;
       vtx->e[2] = eB;
       vtx->sun_w = 0.0f;
-      vtx->dsun_w = 0.0f;
+      vtx->dsun_w[0] = vtx->dsun_w[1] = 0.0f;
       if (shadow) {
-        // the sun term is a * dp1 / pi; its change with the sun elevation at
-        // the same (u, v), by central differences over the offset skies
+        // the sun term is a * dp1 / pi; dp1's derivatives at the same draws
+        // (u, v) in dual numbers through the sun cone sample
         vtx->sun_w = sun_weight(K, pX, pY, pZ, wX, wY, wZ, uv);
-        vtx->dsun_w = (sun_weight(*vtx->sky_e[0], pX, pY, pZ, wX, wY, wZ, uv) -
-                       sun_weight(*vtx->sky_e[1], pX, pY, pZ, wX, wY, wZ, uv)) / (2.0f * vtx->h_e);
+        const Dual3 c = sun_cosine_dual(*vtx->sky, pX, pY, pZ, wX, wY, wZ, uv[0], uv[1]);
+        vtx->dsun_w[0] = static_cast<float>(c.d[kDualElevation] * (1.0 / 3.141592653589793));
+        vtx->dsun_w[1] = static_cast<float>(c.d[kDualAzimuth] * (1.0 / 3.141592653589793));
       }
       vtx->s = 1.0f;
       vtx->uses_albedo = false;

@@ -27,6 +27,7 @@ from .scene import (  # noqa: F401
     build_config,
     camera_setup,
     default_params,
+    diff_checker_map,
     diff_maps,
     pixel_seeds,
     read_scene,

@@ -81,10 +81,13 @@ class Diff(ctypes.Structure):
         ("n_surfaces", ctypes.c_size_t),
         ("albedo", ctypes.POINTER(ctypes.c_float)),
         ("emission", ctypes.POINTER(ctypes.c_float)),
-        ("sky_turbidity", Sky * 2),
-        ("sky_elevation", Sky * 2),
-        ("h_turbidity", ctypes.c_float),
-        ("h_elevation", ctypes.c_float),
+        ("n_checker", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("surface_checker", ctypes.POINTER(ctypes.c_int32)),
+        ("checker", ctypes.POINTER(ctypes.c_float)),
+        ("turbidity", ctypes.c_double),
+        ("elevation_deg", ctypes.c_double),
+        ("azimuth_deg", ctypes.c_double),
     ]
 
 

@@ -281,17 +281,20 @@ class HIPRendererKernel:
         self._check(self._lib.df_reset_counters(self._ctx), "df_reset_counters")
 
     def setDifferentiable(self, surface_albedo, surface_emission, albedo, emission, turbidity, elevation_deg,
-                          azimuth_deg, h_turbidity=None, h_elevation=0.05, sky_active=True, sun_active=True):
+                          azimuth_deg, sky_active=True, sun_active=True, surface_checker=None, checker=None):
         """Differentiable mode: the surfaces mapped by surface_albedo / surface_emission (one entry per surface
-        record, -1 = texture) take the continuous albedo (groups x RGB) and emission (white radiance) parameters;
-        the sky is Sky.set of (turbidity, sun elevation / azimuth in degrees).  The skies at turbidity +-
-        h_turbidity and elevation +- h_elevation give the adjoint its sky derivatives."""
+        record, -1 = texture) take the continuous albedo (groups x RGB) and emission (white radiance) parameters,
+        those mapped by surface_checker the two colours (groups x 2 x RGB) of their checkerboard albedo texture;
+        the sky is Sky.set of (turbidity, sun elevation / azimuth in degrees), differentiated with respect to all
+        three.  Uploads that sky (with the sky / sun switches given) first."""
         from .scene import sky_at
 
+        sc = None if surface_checker is None else np.ascontiguousarray(surface_checker, np.int32)
+        ck = np.ascontiguousarray(checker if checker is not None else np.zeros(0), np.float32).reshape(-1)
         self._diff_keep = [np.ascontiguousarray(surface_albedo, np.int32), np.ascontiguousarray(surface_emission, np.int32),
-                           np.ascontiguousarray(albedo, np.float32).reshape(-1), np.ascontiguousarray(emission, np.float32)]
-        sa, se, al, em = self._diff_keep
-        ht = 1e-3 * float(turbidity) if h_turbidity is None else float(h_turbidity)
+                           np.ascontiguousarray(albedo, np.float32).reshape(-1), np.ascontiguousarray(emission, np.float32),
+                           sc, ck]
+        sa, se, al, em = self._diff_keep[:4]
         d = _ffi.Diff()
         d.n_albedo = al.size // 3
         d.n_emission = em.size
@@ -300,35 +303,34 @@ class HIPRendererKernel:
         d.n_surfaces = sa.size
         d.albedo = al.ctypes.data_as(_ffi.c_float_p)
         d.emission = em.ctypes.data_as(_ffi.c_float_p)
-        d.sky_turbidity[0] = sky_at(elevation_deg, azimuth_deg, turbidity + ht)
-        d.sky_turbidity[1] = sky_at(elevation_deg, azimuth_deg, turbidity - ht)
-        d.sky_elevation[0] = sky_at(elevation_deg + h_elevation, azimuth_deg, turbidity)
-        d.sky_elevation[1] = sky_at(elevation_deg - h_elevation, azimuth_deg, turbidity)
-        d.h_turbidity = ht
-        d.h_elevation = float(h_elevation)
+        d.n_checker = ck.size // 6 if sc is not None else 0
+        if sc is not None:
+            d.surface_checker = sc.ctypes.data_as(_ffi.c_int32_p)
+            d.checker = ck.ctypes.data_as(_ffi.c_float_p)
+        d.turbidity = float(turbidity)
+        d.elevation_deg = float(elevation_deg)
+        d.azimuth_deg = float(azimuth_deg)
         sky = sky_at(elevation_deg, azimuth_deg, turbidity)
         sky.is_sky_active = 1 if sky_active else 0
         sky.is_sun_active = 1 if sun_active else 0
-        for k in (d.sky_turbidity[0], d.sky_turbidity[1], d.sky_elevation[0], d.sky_elevation[1]):
-            k.is_sky_active, k.is_sun_active = sky.is_sky_active, sky.is_sun_active
         self._check(self._lib.df_upload_sky(self._ctx, ctypes.byref(sky)), "df_upload_sky")
         self._check(self._lib.df_set_diff(self._ctx, ctypes.byref(d)), "df_set_diff")
+        self._n_grad = 3 * d.n_albedo + d.n_emission + 6 * d.n_checker + 3
 
     def clearDifferentiable(self):
         self._check(self._lib.df_clear_diff(self._ctx), "df_clear_diff")
 
     def lossGradient(self, target, passes, gradient=True):
         """Forward render of `passes` samples into a cleared film, L = sum (film - target)^2, and with gradient
-        dL/d[albedo groups x RGB, emission, turbidity, elevation] by path replay.
-        Returns (loss, grad or None, forward_ms, adjoint_ms)."""
+        dL/d[albedo groups x RGB, emission, checkerboard groups x 2 x RGB, turbidity, elevation, azimuth] by
+        path replay.  Returns (loss, grad or None, forward_ms, adjoint_ms)."""
         t = np.ascontiguousarray(target, np.float32).reshape(-1)
         loss = ctypes.c_double()
         fms, ams = ctypes.c_float(), ctypes.c_float()
         grad = None
         gp, ng = None, 0
         if gradient:
-            keep = getattr(self, "_diff_keep", None)
-            ng = 3 * (keep[2].size // 3) + keep[3].size + 2 if keep is not None else 2
+            ng = getattr(self, "_n_grad", 3)
             grad = np.zeros(ng, np.float64)
             gp = grad.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         self._check(self._lib.df_loss_gradient(self._ctx, t.ctypes.data_as(_ffi.c_float_p), t.size, int(passes),

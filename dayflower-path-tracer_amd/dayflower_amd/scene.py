@@ -66,6 +66,30 @@ def diff_maps(scene):
     return sa, se, na, ne
 
 
+def diff_checker_map(scene):
+    """Differentiable-mode checkerboard groups of a scene: group g is the g-th surface record, in record order,
+    whose albedo texture is a CheckerboardTexture and that no TriangleMesh primitive uses (those are albedo
+    groups).  Returns (surface_checker, n_checker, colours): one entry per surface record (-1: none) and the
+    groups' texture colours (n_checker x 2 x RGB, colour0 then colour1, each channel byte / 255 as the
+    reference's texture lookup decodes it, CheckerboardTexture.java / GPURendererKernel.java:1714-1767)."""
+    n_surf = scene.surfaces.size // 6
+    sa = diff_maps(scene)[0]
+    sc = np.full(n_surf, -1, np.int32)
+    cols = []
+    recip = np.float32(1.0) / np.float32(255.0)
+    for rec in range(n_surf):
+        off = int(scene.surfaces[rec * 6 + 1])
+        if sa[rec] >= 0 or int(scene.textures[off]) != 3:
+            continue
+        sc[rec] = len(cols)
+        pair = []
+        for k in (2, 3):
+            c = int(np.float32(scene.textures[off + k]))
+            pair.append([np.float32((c >> sh) & 0xFF) * recip for sh in (16, 8, 0)])
+        cols.append(pair)
+    return sc, len(cols), np.array(cols, np.float32).reshape(len(cols), 2, 3)
+
+
 def camera_setup(width, height, fov_x, eye, target, aperture=0.0, focal=30.0):
     """Camera array for a look-at (Camera.java:743-748, 877-946)."""
     cam = np.zeros(19, np.float32)

@@ -167,15 +167,22 @@ const char* df_last_error(df_ctx* ctx);
 /* ---- Differentiable path tracer -------------------------------------------
  * (no counterpart in the reference; planned with the ABI as df_adjoint).
  * df_set_diff switches the path tracer to the parameters of *diff (copied);
- * df_clear_diff restores the textures.  df_loss_gradient renders `passes`
+ * the current sky (df_upload_sky) must be Sky.set of diff's turbidity,
+ * elevation and azimuth (DF_EINVAL otherwise), a checkerboard group may map
+ * only surfaces with a checkerboard albedo texture and no albedo group, and
+ * at most 1024 parameters fit.  df_clear_diff restores the textures, and so
+ * does df_load_scene; df_upload_surfaces refuses to take a checkerboard
+ * group's checkerboard texture away.  df_loss_gradient renders `passes`
  * samples per pixel into a cleared film (the forward pass), returns
  * L = sum (film - target)^2 over the W x H x 3 linear film, and, when grad
- * is not NULL, dL/dparameters (n_grad = 3 n_albedo + n_emission + 2) by
- * replaying every path from the seeds the forward pass started from.  It
- * needs the path tracer, Russian roulette off (pt_depth_russian_roulette >
- * pt_depth_maximum) and pt_depth_maximum <= 8.  forward_ms / adjoint_ms
- * (may be NULL) receive the device times of the two passes.  On return the
- * seeds are those after the forward pass. */
+ * is not NULL, dL/dparameters (n_grad = 3 n_albedo + n_emission +
+ * 6 n_checker + 3, laid out as df_diff describes) by replaying every path
+ * from the seeds the forward pass started from.  It needs the path tracer,
+ * Russian roulette off (pt_depth_russian_roulette > pt_depth_maximum),
+ * pt_depth_maximum <= 8 and the sky df_set_diff was given (DF_ESTATE after
+ * another df_upload_sky).  forward_ms / adjoint_ms (may be NULL) receive the
+ * device times of the two passes.  On return the seeds are those after the
+ * forward pass. */
 int df_set_diff(df_ctx* ctx, const df_diff* diff);
 int df_clear_diff(df_ctx* ctx);
 int df_loss_gradient(df_ctx* ctx, const float* target, size_t n, int32_t passes, double* loss, double* grad,

@@ -163,12 +163,17 @@ typedef struct df_params {
 
 /* Differentiable mode of the path tracer (no counterpart in the reference).
  * Continuous parameters replace the 8-bit albedo / emission textures of the
- * surfaces mapped to them; the sky's turbidity and the sun's elevation are
- * differentiated through skies set up at parameter offsets (Sky.set of the
- * current sky with turbidity +- h_turbidity, and with the sun elevation +-
- * h_elevation degrees at the same azimuth).  Gradient layout: albedo groups
- * (n_albedo x RGB), emission (n_emission), turbidity, sun elevation (per
- * degree). */
+ * surfaces mapped to them: an albedo group's colour replaces the surface's
+ * albedo texture, a checkerboard group's two colours replace those of the
+ * surface's checkerboard albedo texture (whose tile rule stays, without its
+ * single-colour shading quirk), an emission replaces the emission texture.
+ * The sky is differentiated with respect to its turbidity and the sun's
+ * elevation and azimuth: the renderer's current sky must be Sky.set of those
+ * parameters, the sun towards (cos e sin a, sin e, cos e cos a) in the world,
+ * and their derivatives are taken in dual numbers through Sky.set, the sky's
+ * Perez evaluation and the sun cone sample.  Gradient layout (df_loss_gradient):
+ * albedo groups x RGB, emissions, checkerboard groups x 2 colours x RGB,
+ * turbidity, sun elevation, sun azimuth (per degree). */
 typedef struct df_diff {
   int32_t n_albedo;
   int32_t n_emission;
@@ -177,10 +182,13 @@ typedef struct df_diff {
   size_t n_surfaces;
   const float* albedo;              /* n_albedo * 3, linear RGB */
   const float* emission;            /* n_emission, white radiance */
-  df_sky sky_turbidity[2];          /* the sky at turbidity + h, - h */
-  df_sky sky_elevation[2];          /* the sky at sun elevation + h, - h */
-  float h_turbidity;
-  float h_elevation;
+  int32_t n_checker;                /* checkerboard groups; 0: none (surface_checker may be null) */
+  int32_t reserved;
+  const int32_t* surface_checker;   /* per surface record: checkerboard group or -1 */
+  const float* checker;             /* n_checker * 2 * 3: the texture's first colour (dark tiles), its second */
+  double turbidity;
+  double elevation_deg;
+  double azimuth_deg;
 } df_diff;
 
 /* Work counters.  A ray is one intersection query (closest-hit or any-hit);

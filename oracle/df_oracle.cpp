@@ -56,7 +56,8 @@ enum Plant {
   PLANT_PEREZ_THETA0 = 17,     // chromaticity denominators at the sun's zenith angle (no theta-1 quirk)
   PLANT_PEREZ_SWAP = 18,       // luminance Perez coefficients D and E exchanged
   PLANT_FBM_TEXTURE = 19,      // fBm texture noise mapped to [-1, 1] instead of [0, 1]
-  PLANT_CLEARCOAT_OFFSET = 20  // ClearCoat reflection leaves at 0.01 instead of 0.02 along the normal
+  PLANT_CLEARCOAT_OFFSET = 20,  // ClearCoat reflection leaves at 0.01 instead of 0.02 along the normal
+  PLANT_CHECKER_TILE = 21       // differentiable checkerboard group: its two colours swapped on the tiles
 };
 int g_plant = PLANT_NONE;
 inline bool planted(int k) { return g_plant == k; }
@@ -196,53 +197,55 @@ void daylight_xyy_to_srgb(float fx, float fy, double lum, float* r_out, float* g
 // from dual numbers through the sun cone sample at the forward's random
 // numbers: no finite differences anywhere.
 
-// A value with its derivatives with respect to turbidity (t) and sun
-// elevation in degrees (e).
-struct D2 {
-  double v, t, e;
-  D2(double value = 0.0, double dt = 0.0, double de = 0.0) : v(value), t(dt), e(de) {}
+// A value with its derivatives with respect to turbidity (t), sun elevation
+// in degrees (e) and sun azimuth in degrees (a).
+struct D3 {
+  double v, t, e, a;
+  D3(double value = 0.0, double dt = 0.0, double de = 0.0, double da = 0.0) : v(value), t(dt), e(de), a(da) {}
 };
-inline D2 operator+(D2 a, D2 b) { return D2(a.v + b.v, a.t + b.t, a.e + b.e); }
-inline D2 operator-(D2 a, D2 b) { return D2(a.v - b.v, a.t - b.t, a.e - b.e); }
-inline D2 operator-(D2 a) { return D2(-a.v, -a.t, -a.e); }
-inline D2 operator*(D2 a, D2 b) { return D2(a.v * b.v, a.t * b.v + a.v * b.t, a.e * b.v + a.v * b.e); }
-inline D2 operator/(D2 a, D2 b) {
-  const double q = a.v / b.v;
-  return D2(q, (a.t - q * b.t) / b.v, (a.e - q * b.e) / b.v);
+inline D3 operator+(D3 x, D3 y) { return D3(x.v + y.v, x.t + y.t, x.e + y.e, x.a + y.a); }
+inline D3 operator-(D3 x, D3 y) { return D3(x.v - y.v, x.t - y.t, x.e - y.e, x.a - y.a); }
+inline D3 operator-(D3 x) { return D3(-x.v, -x.t, -x.e, -x.a); }
+inline D3 operator*(D3 x, D3 y) {
+  return D3(x.v * y.v, x.t * y.v + x.v * y.t, x.e * y.v + x.v * y.e, x.a * y.v + x.v * y.a);
 }
-inline D2 chain(D2 a, double f, double df) { return D2(f, df * a.t, df * a.e); }
-inline D2 dexp(D2 a) {
+inline D3 operator/(D3 x, D3 y) {
+  const double q = x.v / y.v;
+  return D3(q, (x.t - q * y.t) / y.v, (x.e - q * y.e) / y.v, (x.a - q * y.a) / y.v);
+}
+inline D3 chain(D3 x, double f, double df) { return D3(f, df * x.t, df * x.e, df * x.a); }
+inline D3 dexp(D3 a) {
   const double f = std::exp(a.v);
   return chain(a, f, f);
 }
-inline D2 dsin(D2 a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
-inline D2 dcos(D2 a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
-inline D2 dtan(D2 a) {
+inline D3 dsin(D3 a) { return chain(a, std::sin(a.v), std::cos(a.v)); }
+inline D3 dcos(D3 a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
+inline D3 dtan(D3 a) {
   const double c = std::cos(a.v);
   return chain(a, std::tan(a.v), 1.0 / (c * c));
 }
-inline D2 dsqrt(D2 a) {  // at zero: derivative zero (the cone's centre, where 1 - cos^2 stays zero)
-  if (!(a.v > 0.0)) return D2(0.0);
+inline D3 dsqrt(D3 a) {  // at zero: derivative zero (the cone's centre, where 1 - cos^2 stays zero)
+  if (!(a.v > 0.0)) return D3(0.0);
   const double f = std::sqrt(a.v);
   return chain(a, f, 0.5 / f);
 }
-inline D2 dacos_clamped(D2 a) {  // acos(max(min(a, 1), -1)): constant outside [-1, 1]
-  if (a.v >= 1.0) return D2(0.0);
-  if (a.v <= -1.0) return D2(M_PI);
+inline D3 dacos_clamped(D3 a) {  // acos(max(min(a, 1), -1)): constant outside [-1, 1]
+  if (a.v >= 1.0) return D3(0.0);
+  if (a.v <= -1.0) return D3(M_PI);
   return chain(a, std::acos(a.v), -1.0 / std::sqrt(1.0 - a.v * a.v));
 }
-inline D2 dmax0(D2 a) { return a.v > 0.0 ? a : D2(0.0); }
+inline D3 dmax0(D3 a) { return a.v > 0.0 ? a : D3(0.0); }
 
-// Sky.set (Sky.java:295-333) as a function of turbidity T and sun elevation
-// e (degrees) at a fixed azimuth a, the sun towards (cos e sin a, sin e,
+// Sky.set (Sky.java:295-333) as a function of turbidity T, sun elevation e
+// and sun azimuth a (degrees), the sun towards (cos e sin a, sin e,
 // cos e cos a) in the world (y up, azimuth from +z towards +x).  The sky
 // frame is the one the forward's sky holds (u = x, v = z, w = y).
 struct DualSky {
-  D2 world[3];  // unit vector towards the sun, world frame
-  D2 dir[3];    // the same in the sky frame
-  D2 theta;     // the sun's zenith angle
-  D2 zl, zx, zy;
-  D2 L[5], X[5], Y[5];
+  D3 world[3];  // unit vector towards the sun, world frame
+  D3 dir[3];    // the same in the sky frame
+  D3 theta;     // the sun's zenith angle
+  D3 zl, zx, zy;
+  D3 L[5], X[5], Y[5];
   float onb_u[3], onb_v[3], onb_w[3];
 };
 
@@ -252,26 +255,26 @@ DualSky dual_sky(const df_sky* frame, double turbidity, double elevation_deg, do
   std::memcpy(K.onb_v, frame->onb_v, sizeof K.onb_v);
   std::memcpy(K.onb_w, frame->onb_w, sizeof K.onb_w);
   const double rad = M_PI / 180.0;
-  const D2 el(elevation_deg * rad, 0.0, rad);
-  const D2 T(turbidity, 1.0, 0.0);
-  const double sa = std::sin(azimuth_deg * rad), ca = std::cos(azimuth_deg * rad);
-  K.world[0] = dcos(el) * sa;
+  const D3 el(elevation_deg * rad, 0.0, rad, 0.0);
+  const D3 az(azimuth_deg * rad, 0.0, 0.0, rad);
+  const D3 T(turbidity, 1.0, 0.0, 0.0);
+  K.world[0] = dcos(el) * dsin(az);
   K.world[1] = dsin(el);
-  K.world[2] = dcos(el) * ca;
+  K.world[2] = dcos(el) * dcos(az);
   const float* onb[3] = {K.onb_u, K.onb_v, K.onb_w};
   for (int i = 0; i < 3; ++i) K.dir[i] = K.world[0] * onb[i][0] + K.world[1] * onb[i][1] + K.world[2] * onb[i][2];
-  const D2 len = dsqrt(K.dir[0] * K.dir[0] + K.dir[1] * K.dir[1] + K.dir[2] * K.dir[2]);
-  for (D2& c : K.dir) c = c / len;
+  const D3 len = dsqrt(K.dir[0] * K.dir[0] + K.dir[1] * K.dir[1] + K.dir[2] * K.dir[2]);
+  for (D3& c : K.dir) c = c / len;
   K.theta = dacos_clamped(K.dir[2]);
-  const D2 th = K.theta, th2 = th * th, th3 = th2 * th, T2 = T * T;
+  const D3 th = K.theta, th2 = th * th, th3 = th2 * th, T2 = T * T;
   const double pi_f = static_cast<double>(static_cast<float>(M_PI));  // MathF.PI
-  const D2 chi = (D2(4.0 / 9.0) - T / D2(120.0)) * (D2(pi_f) - D2(2.0) * th);
-  K.zl = ((D2(4.0453) * T - D2(4.9710)) * dtan(chi) - D2(0.2155) * T + D2(2.4192)) * D2(1000.0);
+  const D3 chi = (D3(4.0 / 9.0) - T / D3(120.0)) * (D3(pi_f) - D3(2.0) * th);
+  K.zl = ((D3(4.0453) * T - D3(4.9710)) * dtan(chi) - D3(0.2155) * T + D3(2.4192)) * D3(1000.0);
   auto poly = [&](const double c[12]) {  // (c0 th3 + c1 th2 + c2 th + c3) T2 + (...) T + (...)
-    D2 r(0.0);
-    const D2 coef[3] = {T2, T, D2(1.0)};
+    D3 r(0.0);
+    const D3 coef[3] = {T2, T, D3(1.0)};
     for (int k = 0; k < 3; ++k)
-      r = r + (D2(c[4 * k]) * th3 + D2(c[4 * k + 1]) * th2 + D2(c[4 * k + 2]) * th + D2(c[4 * k + 3])) * coef[k];
+      r = r + (D3(c[4 * k]) * th3 + D3(c[4 * k + 1]) * th2 + D3(c[4 * k + 2]) * th + D3(c[4 * k + 3])) * coef[k];
     return r;
   };
   const double cx[12] = {0.00165, -0.00374, 0.00208, 0.0, -0.02902, 0.06377, -0.03202, 0.00394,
@@ -288,9 +291,9 @@ DualSky dual_sky(const df_sky* frame, double turbidity, double elevation_deg, do
   const double py[5][2] = {{-0.01669, -0.26078}, {-0.09495, 0.00921}, {-0.00792, 0.21023}, {-0.04405, -1.65369},
                            {-0.01092, 0.05291}};
   for (int i = 0; i < 5; ++i) {
-    K.L[i] = D2(pl[i][0]) * T + D2(pl[i][1]);
-    K.X[i] = D2(px[i][0]) * T + D2(px[i][1]);
-    K.Y[i] = D2(py[i][0]) * T + D2(py[i][1]);
+    K.L[i] = D3(pl[i][0]) * T + D3(pl[i][1]);
+    K.X[i] = D3(px[i][0]) * T + D3(px[i][1]);
+    K.Y[i] = D3(py[i][0]) * T + D3(py[i][1]);
   }
   return K;
 }
@@ -298,7 +301,7 @@ DualSky dual_sky(const df_sky* frame, double turbidity, double elevation_deg, do
 // doCalculateColorForSky (GPURendererKernel.java:2127-2236) in dual numbers
 // for the direction (dx, dy, dz), which does not depend on the parameters.
 // Returns false below the horizon, where the sky is a constant.
-bool dual_sky_rgb(const DualSky& K, float dx, float dy, float dz, D2 out[3]) {
+bool dual_sky_rgb(const DualSky& K, float dx, float dy, float dz, D3 out[3]) {
   const float d0X = dx * K.onb_u[0] + dy * K.onb_u[1] + dz * K.onb_u[2];
   const float d0Y = dx * K.onb_v[0] + dy * K.onb_v[1] + dz * K.onb_v[2];
   float d0Z = dx * K.onb_w[0] + dy * K.onb_w[1] + dz * K.onb_w[2];
@@ -307,44 +310,44 @@ bool dual_sky_rgb(const DualSky& K, float dx, float dy, float dz, D2 out[3]) {
   const double l = 1.0 / std::sqrt(static_cast<double>(d0X) * d0X + static_cast<double>(d0Y) * d0Y +
                                    static_cast<double>(d0Z) * d0Z);
   const double d1X = d0X * l, d1Y = d0Y * l, d1Z = d0Z * l;
-  const D2 dp = K.dir[0] * D2(d1X) + K.dir[1] * D2(d1Y) + K.dir[2] * D2(d1Z);
+  const D3 dp = K.dir[0] * D3(d1X) + K.dir[1] * D3(d1Y) + K.dir[2] * D3(d1Z);
   const double theta1 = std::acos(jmaxd(jmind(d1Z, 1.0), -1.0));
   const double cosTheta1R = 1.0 / (std::cos(theta1) + 0.01);
-  const D2 gamma = dacos_clamped(dp);
-  const D2 cosGamma = dcos(gamma);
-  const D2 cosTheta0 = dcos(K.theta);
-  const D2 one(1.0);
-  auto f_sun = [&](const D2* C, D2 angle) {  // 1 + C2 exp(C3 angle) + C4 cos^2
+  const D3 gamma = dacos_clamped(dp);
+  const D3 cosGamma = dcos(gamma);
+  const D3 cosTheta0 = dcos(K.theta);
+  const D3 one(1.0);
+  auto f_sun = [&](const D3* C, D3 angle) {  // 1 + C2 exp(C3 angle) + C4 cos^2
     return one + C[2] * dexp(C[3] * angle) + C[4] * cosGamma * cosGamma;
   };
-  const D2* L = K.L;
-  const D2* X = K.X;
-  const D2* Y = K.Y;
-  const D2 lden = (one + L[0] * dexp(L[1])) * (one + L[2] * dexp(L[3] * K.theta) + L[4] * cosTheta0 * cosTheta0);
-  const D2 lnum = (one + L[0] * dexp(L[1] * D2(cosTheta1R))) * f_sun(L, gamma);
-  const D2 lum = K.zl * lnum / lden * D2(1.0e-4);
-  const D2 xden = (one + X[0] * dexp(X[1])) * (one + X[2] * dexp(X[3] * D2(theta1)) + X[4] * cosTheta0 * cosTheta0);
-  const D2 xnum = (one + X[0] * dexp(X[1] * D2(cosTheta1R))) * f_sun(X, gamma);
-  const D2 x = K.zx * xnum / xden;
-  const D2 yden = (one + Y[0] * dexp(Y[1])) * (one + Y[2] * dexp(Y[3] * D2(theta1)) + Y[4] * cosTheta0 * cosTheta0);
-  const D2 ynum = (one + Y[0] * dexp(Y[1] * D2(cosTheta1R))) * f_sun(Y, gamma);
-  const D2 y = K.zy * ynum / yden;
+  const D3* L = K.L;
+  const D3* X = K.X;
+  const D3* Y = K.Y;
+  const D3 lden = (one + L[0] * dexp(L[1])) * (one + L[2] * dexp(L[3] * K.theta) + L[4] * cosTheta0 * cosTheta0);
+  const D3 lnum = (one + L[0] * dexp(L[1] * D3(cosTheta1R))) * f_sun(L, gamma);
+  const D3 lum = K.zl * lnum / lden * D3(1.0e-4);
+  const D3 xden = (one + X[0] * dexp(X[1])) * (one + X[2] * dexp(X[3] * D3(theta1)) + X[4] * cosTheta0 * cosTheta0);
+  const D3 xnum = (one + X[0] * dexp(X[1] * D3(cosTheta1R))) * f_sun(X, gamma);
+  const D3 x = K.zx * xnum / xden;
+  const D3 yden = (one + Y[0] * dexp(Y[1])) * (one + Y[2] * dexp(Y[3] * D3(theta1)) + Y[4] * cosTheta0 * cosTheta0);
+  const D3 ynum = (one + Y[0] * dexp(Y[1] * D3(cosTheta1R))) * f_sun(Y, gamma);
+  const D3 y = K.zy * ynum / yden;
   // daylight chromaticity -> linear sRGB (GPURendererKernel.java:2207-2230)
-  const D2 cie0 = one / (D2(0.0241) + D2(0.2562) * x - D2(0.7341) * y);
-  const D2 cie1 = (D2(-1.3515) - D2(1.7703) * x + D2(5.9114) * y) * cie0;
-  const D2 cie2 = (D2(0.03) - D2(31.4424) * x + D2(30.0717) * y) * cie0;
-  const D2 cX = D2(10246.121) + cie1 * D2(187.75537) + cie2 * D2(213.14803);
-  const D2 cY = D2(10676.695) + cie1 * D2(192.59653) + cie2 * D2(76.29494);
-  const D2 cZ = D2(12372.504) + cie1 * D2(3482.8765) + cie2 * D2(-235.71611);
-  const D2 cR = cX * lum / cY, cG = lum, cB = cZ * lum / cY;
-  const D2 r = D2(3.2410042) * cR + D2(-1.5373994) * cG + D2(-0.49861607) * cB;
-  const D2 g = D2(-0.9692241) * cR + D2(1.8759298) * cG + D2(0.041554242) * cB;
-  const D2 b = D2(0.05563942) * cR + D2(-0.20401107) * cG + D2(1.0571486) * cB;
+  const D3 cie0 = one / (D3(0.0241) + D3(0.2562) * x - D3(0.7341) * y);
+  const D3 cie1 = (D3(-1.3515) - D3(1.7703) * x + D3(5.9114) * y) * cie0;
+  const D3 cie2 = (D3(0.03) - D3(31.4424) * x + D3(30.0717) * y) * cie0;
+  const D3 cX = D3(10246.121) + cie1 * D3(187.75537) + cie2 * D3(213.14803);
+  const D3 cY = D3(10676.695) + cie1 * D3(192.59653) + cie2 * D3(76.29494);
+  const D3 cZ = D3(12372.504) + cie1 * D3(3482.8765) + cie2 * D3(-235.71611);
+  const D3 cR = cX * lum / cY, cG = lum, cB = cZ * lum / cY;
+  const D3 r = D3(3.2410042) * cR + D3(-1.5373994) * cG + D3(-0.49861607) * cB;
+  const D3 g = D3(-0.9692241) * cR + D3(1.8759298) * cG + D3(0.041554242) * cB;
+  const D3 b = D3(0.05563942) * cR + D3(-0.20401107) * cG + D3(1.0571486) * cB;
   // w = max(0, -min(0, min(r, g, b))): the smallest channel lifted to zero
-  const D2* lo = &r;
+  const D3* lo = &r;
   if (g.v < lo->v) lo = &g;
   if (b.v < lo->v) lo = &b;
-  const D2 w = lo->v < 0.0 ? -*lo : D2(0.0);
+  const D3 w = lo->v < 0.0 ? -*lo : D3(0.0);
   out[0] = r + w;
   out[1] = g + w;
   out[2] = b + w;
@@ -354,37 +357,43 @@ bool dual_sky_rgb(const DualSky& K, float dx, float dy, float dz, D2 out[3]) {
 // The cosine dp1 between the normal (nX, nY, nZ) and the sun direction that
 // doCalculateColorForSun (GPURendererKernel.java:2238-2347) samples with the
 // random numbers (u, v) at (pX, pY, pZ), as a function of the sun's
-// elevation (the sun's origin sits 10000 units along its direction).
-D2 dual_sun_cosine(const DualSky& K, float pX, float pY, float pZ, float nX, float nY, float nZ, float u, float v) {
-  const D2* w = K.world;
-  const D2 toX = w[0] * D2(10000.0) - D2(pX), toY = w[1] * D2(10000.0) - D2(pY), toZ = w[2] * D2(10000.0) - D2(pZ);
-  const D2 to2 = toX * toX + toY * toY + toZ * toZ;
-  const D2 sinMax2 = D2(500.0 * 500.0) / to2;
-  const D2 cosMax = dsqrt(dmax0(D2(1.0) - sinMax2));
-  const D2 cosT = D2(u) * (cosMax - D2(1.0)) + D2(1.0);
-  const D2 sinT = dsqrt(dmax0(D2(1.0) - cosT * cosT));
+// elevation and azimuth (the sun's origin sits 10000 units along its
+// direction).
+D3 dual_sun_cosine(const DualSky& K, float pX, float pY, float pZ, float nX, float nY, float nZ, float u, float v) {
+  const D3* w = K.world;
+  const D3 toX = w[0] * D3(10000.0) - D3(pX), toY = w[1] * D3(10000.0) - D3(pY), toZ = w[2] * D3(10000.0) - D3(pZ);
+  const D3 to2 = toX * toX + toY * toY + toZ * toZ;
+  const D3 sinMax2 = D3(500.0 * 500.0) / to2;
+  const D3 cosMax = dsqrt(dmax0(D3(1.0) - sinMax2));
+  const D3 cosT = D3(u) * (cosMax - D3(1.0)) + D3(1.0);
+  const D3 sinT = dsqrt(dmax0(D3(1.0) - cosT * cosT));
   const float phi = PI_MULTIPLIED_BY_TWO * v;
-  const D2 x = D2(std::cos(static_cast<double>(phi))) * sinT;
-  const D2 y = D2(std::sin(static_cast<double>(phi))) * sinT;
-  const D2 z = cosT;
+  const D3 x = D3(std::cos(static_cast<double>(phi))) * sinT;
+  const D3 y = D3(std::sin(static_cast<double>(phi))) * sinT;
+  const D3 z = cosT;
   const double aX = std::fabs(w[0].v), aY = std::fabs(w[1].v), aZ = std::fabs(w[2].v);
   const bool c0 = aX < aY && aX < aZ;
   const bool c1 = aY < aZ;
-  const D2 vX = c0 ? D2(0.0) : c1 ? w[2] : w[1];
-  const D2 vY = c0 ? w[2] : c1 ? D2(0.0) : -w[0];
-  const D2 vZ = c0 ? -w[1] : c1 ? -w[0] : D2(0.0);
-  const D2 vl = dsqrt(vX * vX + vY * vY + vZ * vZ);
-  const D2 vnX = vX / vl, vnY = vY / vl, vnZ = vZ / vl;
-  const D2 unX = vnY * w[2] - vnZ * w[1], unY = vnZ * w[0] - vnX * w[2], unZ = vnX * w[1] - vnY * w[0];
-  const D2 rX = unX * x + vnX * y + w[0] * z;
-  const D2 rY = unY * x + vnY * y + w[1] * z;
-  const D2 rZ = unZ * x + vnZ * y + w[2] * z;
-  const D2 rl = dsqrt(rX * rX + rY * rY + rZ * rZ);
-  return (rX * D2(nX) + rY * D2(nY) + rZ * D2(nZ)) / rl;
+  const D3 vX = c0 ? D3(0.0) : c1 ? w[2] : w[1];
+  const D3 vY = c0 ? w[2] : c1 ? D3(0.0) : -w[0];
+  const D3 vZ = c0 ? -w[1] : c1 ? -w[0] : D3(0.0);
+  const D3 vl = dsqrt(vX * vX + vY * vY + vZ * vZ);
+  const D3 vnX = vX / vl, vnY = vY / vl, vnZ = vZ / vl;
+  const D3 unX = vnY * w[2] - vnZ * w[1], unY = vnZ * w[0] - vnX * w[2], unZ = vnX * w[1] - vnY * w[0];
+  const D3 rX = unX * x + vnX * y + w[0] * z;
+  const D3 rY = unY * x + vnY * y + w[1] * z;
+  const D3 rZ = unZ * x + vnZ * y + w[2] * z;
+  const D3 rl = dsqrt(rX * rX + rY * rY + rZ * rZ);
+  return (rX * D3(nX) + rY * D3(nY) + rZ * D3(nZ)) / rl;
 }
 
 // Differentiable mode: the surface records mapped to a parameter take it in
-// place of their constant texture.
+// place of their texture.  An albedo group replaces the albedo texture by one
+// colour; a checkerboard group replaces the two colours of a checkerboard
+// albedo texture, the tile rule staying the texture's.  Albedo slots number
+// the colours of both: slot k < n_albedo is group k, slot n_albedo + 2 g + t
+// is colour t (0: the texture's first colour, used on dark tiles) of
+// checkerboard group g.
 struct DiffMaps {
   int n_albedo, n_emission;
   const int32_t* surface_albedo;
@@ -392,22 +401,29 @@ struct DiffMaps {
   int n_surfaces;
   const float* albedo;
   const float* emission;
+  int n_checker;
+  const int32_t* surface_checker;  // null: no checkerboard groups
+  const float* checker;            // n_checker x 2 x RGB
 };
 
 // The tangent of one path: its throughput in double, the derivative of the
-// throughput with respect to the albedo groups the path has met so far
-// (the only parameters it depends on), and the pixel's running sum of the
-// derivatives of its samples, over all parameters.
+// throughput with respect to the albedo slots the path has met so far (the
+// only parameters it depends on), and the pixel's running sum of the
+// derivatives of its samples, over all parameters.  Gradient rows: albedo
+// groups x RGB, emissions, checkerboard groups x 2 x RGB, turbidity, sun
+// elevation, sun azimuth.
 struct PathTangent {
   static constexpr int kMaxSlots = 64;
   const DualSky* sky;
-  int n_albedo, i_emission, i_turbidity, i_elevation;
+  int n_albedo, n_emission, i_emission, i_turbidity, i_elevation, i_azimuth;
   double m[3];
   int slots[kMaxSlots];
   double dm[kMaxSlots][3];
   int n_slots;
-  double* dpix;  // (3 n_albedo + n_emission + 2) x 3
+  double* dpix;  // (3 n_albedo + n_emission + 6 n_checker + 3) x 3
 
+  // the gradient row of channel 0 of albedo slot `slot`
+  int row(int slot) const { return 3 * slot + (slot >= n_albedo ? n_emission : 0); }
   void begin() {
     m[0] = m[1] = m[2] = 1.0;
     n_slots = 0;
@@ -422,7 +438,7 @@ struct PathTangent {
   // d(sample) += d(m) * g: the term m * g with g not depending on the albedos
   void through_throughput(const double g[3]) {
     for (int k = 0; k < n_slots; ++k)
-      for (int c = 0; c < 3; ++c) dpix[(slots[k] * 3 + c) * 3 + c] += dm[k][c] * g[c];
+      for (int c = 0; c < 3; ++c) dpix[(row(slots[k]) + c) * 3 + c] += dm[k][c] * g[c];
   }
   // m *= s, s free of parameters
   void scale(float s) {
@@ -430,7 +446,7 @@ struct PathTangent {
     for (int k = 0; k < n_slots; ++k)
       for (int c = 0; c < 3; ++c) dm[k][c] *= s;
   }
-  // m *= a, a the albedo of group `slot` (-1: a texture, no parameter)
+  // m *= a, a the albedo of slot `slot` (-1: a texture, no parameter)
   void albedo(int slot, const float a[3]) {
     for (int k = 0; k < n_slots; ++k)
       for (int c = 0; c < 3; ++c) dm[k][c] *= a[c];
@@ -625,6 +641,7 @@ class Item {
   int tex_blend(int off);
   int tex_bullseye(int off);
   int tex_checker(int off);
+  int checker_tile(int off);
   int tex_fbm(int off);
   int tex_image(int off);
   int tex_image_simple(int off);
@@ -1344,20 +1361,27 @@ int Item::tex_bullseye(int off) {  // :1668-1712
                   saturate(cB, 0.0f, 1.0f) * 255.0f + 0.5f);
 }
 
-int Item::tex_checker(int off) {  // :1714-1767
+// The tile of the checkerboard texture at `off` under the hit record's uv
+// (:1741-1747): 0 where the texture takes its first colour, 1 the second.
+int Item::checker_tile(int off) {
   const float* T = s_.a->textures;
   const float u = rec_[R_UV], v = rec_[R_UV + 1];
+  const float sU = T[off + 6], sV = T[off + 7];
+  const float ca = T[off + 4], sa = T[off + 5];
+  const float tu = modulo((u * ca - v * sa) * sU);
+  const float tv = modulo((v * ca + u * sa) * sV);
+  return (tu > 0.5f) != (tv > 0.5f) ? 0 : 1;
+}
+
+int Item::tex_checker(int off) {  // :1714-1767
+  const float* T = s_.a->textures;
   const int c0 = jf2i(T[off + 2]);
   const int c1 = jf2i(T[off + 3]);
   const float c0R = ((c0 >> 16) & 0xFF) * COLOR_RECIPROCAL, c0G = ((c0 >> 8) & 0xFF) * COLOR_RECIPROCAL,
               c0B = (c0 & 0xFF) * COLOR_RECIPROCAL;
   const float c1R = ((c1 >> 16) & 0xFF) * COLOR_RECIPROCAL, c1G = ((c1 >> 8) & 0xFF) * COLOR_RECIPROCAL,
               c1B = (c1 & 0xFF) * COLOR_RECIPROCAL;
-  const float sU = T[off + 6], sV = T[off + 7];
-  const float ca = T[off + 4], sa = T[off + 5];
-  const float tu = modulo((u * ca - v * sa) * sU);
-  const float tv = modulo((v * ca + u * sa) * sV);
-  const bool dark = (tu > 0.5f) != (tv > 0.5f);
+  const bool dark = checker_tile(off) == 0;
   const float m = dark ? 0.8f : 1.2f;
   float cR = dark ? c0R : c1R;
   float cG = dark ? c0G : c1G;
@@ -1694,6 +1718,12 @@ void Item::render_pt(bool sky_active, bool sun_active) {  // :3151-3705
         aR = s_.diff->albedo[aslot * 3 + 0];
         aG = s_.diff->albedo[aslot * 3 + 1];
         aB = s_.diff->albedo[aslot * 3 + 2];
+      } else if (s_.diff->surface_checker && s_.diff->surface_checker[so / 6] >= 0) {
+        const int k = 2 * s_.diff->surface_checker[so / 6] + (checker_tile(ta) ^ (planted(PLANT_CHECKER_TILE) ? 1 : 0));
+        aslot = s_.diff->n_albedo + k;
+        aR = s_.diff->checker[k * 3 + 0];
+        aG = s_.diff->checker[k * 3 + 1];
+        aB = s_.diff->checker[k * 3 + 2];
       }
       if (eslot >= 0) eR = eG = eB = s_.diff->emission[eslot];
     }
@@ -1946,23 +1976,26 @@ void Item::scatter(int mat, int aslot, float pX, float pY, float pZ, float sX, f
 }
 
 // The sky term m * sky(d) / pi of a path that left the scene: through the
-// albedos in m, and through turbidity and elevation in the sky radiance.
+// albedos in m, and through turbidity, sun elevation and sun azimuth in the
+// sky radiance.
 void Item::tan_sky(bool active, float dx, float dy, float dz) {
   PathTangent& T = *tan_;
   const double g[3] = {static_cast<double>(tmp_[0]) * PI_RECIPROCAL, static_cast<double>(tmp_[1]) * PI_RECIPROCAL,
                        static_cast<double>(tmp_[2]) * PI_RECIPROCAL};
   T.through_throughput(g);
-  D2 rgb[3];
+  D3 rgb[3];
   if (!active || !dual_sky_rgb(*T.sky, dx, dy, dz, rgb)) return;
   for (int c = 0; c < 3; ++c) {
     T.dpix[T.i_turbidity * 3 + c] += T.m[c] * rgb[c].t * PI_RECIPROCAL;
     T.dpix[T.i_elevation * 3 + c] += T.m[c] * rgb[c].e * PI_RECIPROCAL;
+    T.dpix[T.i_azimuth * 3 + c] += T.m[c] * rgb[c].a * PI_RECIPROCAL;
   }
 }
 
 // The terms m * (e + sun) of a surface vertex: through the albedos in m, the
 // light's own emission parameter, and the sun term a * dp1 / pi through the
-// vertex's albedo and, at the sample's random numbers, the sun's elevation.
+// vertex's albedo and, at the sample's random numbers, the sun's elevation
+// and azimuth.
 void Item::tan_vertex(int aslot, int eslot, const float a[3], const float e[3], bool sun_term, float pX, float pY,
                       float pZ, float wX, float wY, float wZ) {
   PathTangent& T = *tan_;
@@ -1973,9 +2006,12 @@ void Item::tan_vertex(int aslot, int eslot, const float a[3], const float e[3], 
     for (int c = 0; c < 3; ++c) T.dpix[(T.i_emission + eslot) * 3 + c] += T.m[c];
   if (!sun_term || !sun_lit_) return;
   if (aslot >= 0)
-    for (int c = 0; c < 3; ++c) T.dpix[(aslot * 3 + c) * 3 + c] += T.m[c] * sun_dp1_ * PI_RECIPROCAL;
-  const D2 dp1 = dual_sun_cosine(*T.sky, pX, pY, pZ, wX, wY, wZ, sun_u_, sun_v_);
-  for (int c = 0; c < 3; ++c) T.dpix[T.i_elevation * 3 + c] += T.m[c] * a[c] * dp1.e * PI_RECIPROCAL;
+    for (int c = 0; c < 3; ++c) T.dpix[(T.row(aslot) + c) * 3 + c] += T.m[c] * sun_dp1_ * PI_RECIPROCAL;
+  const D3 dp1 = dual_sun_cosine(*T.sky, pX, pY, pZ, wX, wY, wZ, sun_u_, sun_v_);
+  for (int c = 0; c < 3; ++c) {
+    T.dpix[T.i_elevation * 3 + c] += T.m[c] * a[c] * dp1.e * PI_RECIPROCAL;
+    T.dpix[T.i_azimuth * 3 + c] += T.m[c] * a[c] * dp1.a * PI_RECIPROCAL;
+  }
 }
 
 void Item::render_rc(bool sky_active) {  // :3707-3800
@@ -2318,8 +2354,9 @@ int oracle_render(const df_compiled_scene* scene, const df_sky* sky, const df_pa
 
 // Differentiable mode of the path tracer (no counterpart in the reference):
 // the surface records mapped to continuous parameters (albedo group x RGB,
-// white emission; -1 keeps the texture), the parameter values, and the
-// sky's parameters (the df_sky rendered with is Sky.set of them).
+// white emission, checkerboard group x 2 colours x RGB; -1 keeps the
+// texture), the parameter values, and the sky's parameters (the df_sky
+// rendered with is Sky.set of them).
 typedef struct oracle_diff {
   int32_t n_albedo, n_emission;
   const int32_t* surface_albedo;    // per surface record
@@ -2327,16 +2364,21 @@ typedef struct oracle_diff {
   int64_t n_surfaces;
   const float* albedo;    // n_albedo x 3
   const float* emission;  // n_emission
+  int32_t n_checker;
+  const int32_t* surface_checker;  // per surface record, or null (no checkerboard groups)
+  const float* checker;            // n_checker x 2 x 3: the first colour (dark tiles), the second
   double turbidity, elevation_deg, azimuth_deg;
 } oracle_diff;
 
 // L = sum over the crop [x0, x0 + crop_w) x [y0, y0 + crop_h) of
 // (film - target)^2, the film being the mean of `passes` path-traced samples
 // per pixel from the starting seeds given (width x height, not modified), and
-// its gradient dL/d[albedo groups x RGB, emission, turbidity, elevation per
-// degree] by forward-mode differentiation in double.  Needs the path tracer
-// with Russian roulette off, no wireframes and depth <= 64.  Writes the
-// crop's film (crop_w x crop_h x 3).  Returns 0 or DF_EINVAL.
+// its gradient dL/d[albedo groups x RGB, emission, checkerboard groups x 2 x
+// RGB, turbidity, sun elevation per degree, sun azimuth per degree] by
+// forward-mode differentiation in double.  Needs the path tracer with Russian
+// roulette off, no wireframes and depth <= 64.  A surface takes at most one
+// albedo map, and a checkerboard group only a checkerboard albedo texture.
+// Writes the crop's film (crop_w x crop_h x 3).  Returns 0 or DF_EINVAL.
 int oracle_loss_gradient(const df_compiled_scene* scene, const df_sky* sky, const df_params* params,
                          const oracle_diff* diff, int32_t width, int32_t height, int32_t x0, int32_t y0,
                          int32_t crop_w, int32_t crop_h, const int64_t* seeds, const float* target, int passes,
@@ -2350,19 +2392,27 @@ int oracle_loss_gradient(const df_compiled_scene* scene, const df_sky* sky, cons
     return DF_EINVAL;
   if (diff->n_albedo < 0 || diff->n_emission < 0 || diff->n_surfaces != static_cast<int64_t>(scene->surfaces_len / 6))
     return DF_EINVAL;
-  const int n = 3 * diff->n_albedo + diff->n_emission + 2;
+  const int n_checker = diff->surface_checker ? diff->n_checker : 0;
+  if (n_checker < 0 || (n_checker > 0 && !diff->checker)) return DF_EINVAL;
+  const int n = 3 * diff->n_albedo + diff->n_emission + 6 * n_checker + 3;
   if (n_grad != n) return DF_EINVAL;
   for (int64_t i = 0; i < diff->n_surfaces; ++i) {
     if (diff->surface_albedo[i] < -1 || diff->surface_albedo[i] >= diff->n_albedo) return DF_EINVAL;
     if (diff->surface_emission[i] < -1 || diff->surface_emission[i] >= diff->n_emission) return DF_EINVAL;
+    if (!diff->surface_checker) continue;
+    const int g = diff->surface_checker[i];
+    if (g < -1 || g >= n_checker) return DF_EINVAL;
+    if (g >= 0 && (diff->surface_albedo[i] >= 0 || jf2i(scene->textures[jf2i(scene->surfaces[i * 6 + 1])]) != 3))
+      return DF_EINVAL;
   }
   const DualSky K = dual_sky(sky, diff->turbidity, diff->elevation_deg, diff->azimuth_deg);
   for (int i = 0; i < 3; ++i)  // the sky must be the one these parameters set up
     if (std::fabs(K.world[i].v - sky->sun_direction_world[i]) > 1e-5) return DF_EINVAL;
   if (std::fabs(diff->turbidity - sky->turbidity) > 1e-5 * diff->turbidity) return DF_EINVAL;
 
-  const DiffMaps maps{diff->n_albedo, diff->n_emission, diff->surface_albedo, diff->surface_emission,
-                      static_cast<int>(diff->n_surfaces), diff->albedo, diff->emission};
+  const DiffMaps maps{diff->n_albedo,   diff->n_emission, diff->surface_albedo, diff->surface_emission,
+                      static_cast<int>(diff->n_surfaces), diff->albedo,     diff->emission,       n_checker,
+                      n_checker > 0 ? diff->surface_checker : nullptr,      diff->checker};
   Scene s{scene, sky, params, static_cast<int>(scene->primitives_len / 3), &maps};
   const size_t npix = static_cast<size_t>(width) * height;
   std::vector<float> film(npix * 3, 0.0f);
@@ -2381,9 +2431,11 @@ int oracle_loss_gradient(const df_compiled_scene* scene, const df_sky* sky, cons
       PathTangent pt;
       pt.sky = &K;
       pt.n_albedo = diff->n_albedo;
+      pt.n_emission = diff->n_emission;
       pt.i_emission = 3 * diff->n_albedo;
-      pt.i_turbidity = n - 2;
-      pt.i_elevation = n - 1;
+      pt.i_turbidity = n - 3;
+      pt.i_elevation = n - 2;
+      pt.i_azimuth = n - 1;
       pt.dpix = dpix.data();
       std::vector<double>& g = grads[t];
       for (int y = y0 + crop_h * t / threads; y < y0 + crop_h * (t + 1) / threads; ++y) {
@@ -2421,28 +2473,32 @@ int oracle_loss_gradient(const df_compiled_scene* scene, const df_sky* sky, cons
 
 // The dual-number sky of the FP64 adjoint towards d: out[c] the radiance,
 // out[3 + c] its derivative with respect to turbidity, out[6 + c] with
-// respect to the sun elevation in degrees.  Returns 0 below the horizon.
+// respect to the sun elevation in degrees, out[9 + c] to the sun azimuth in
+// degrees.  Returns 0 below the horizon.
 int oracle_sky_dual(const df_sky* frame, double turbidity, double elevation_deg, double azimuth_deg, const float d[3],
-                    double out[9]) {
+                    double out[12]) {
   const DualSky K = dual_sky(frame, turbidity, elevation_deg, azimuth_deg);
-  D2 rgb[3];
+  D3 rgb[3];
   if (!dual_sky_rgb(K, d[0], d[1], d[2], rgb)) return 0;
   for (int c = 0; c < 3; ++c) {
     out[c] = rgb[c].v;
     out[3 + c] = rgb[c].t;
     out[6 + c] = rgb[c].e;
+    out[9 + c] = rgb[c].a;
   }
   return 1;
 }
 
 // The dual-number sun cosine of the FP64 adjoint: out[0] the cosine, out[1]
-// its derivative with respect to the sun elevation in degrees.
+// its derivative with respect to the sun elevation in degrees, out[2] to the
+// sun azimuth in degrees.
 void oracle_sun_cosine_dual(const df_sky* frame, double turbidity, double elevation_deg, double azimuth_deg,
-                            const float p[3], const float n[3], float u, float v, double out[2]) {
+                            const float p[3], const float n[3], float u, float v, double out[3]) {
   const DualSky K = dual_sky(frame, turbidity, elevation_deg, azimuth_deg);
-  const D2 c = dual_sun_cosine(K, p[0], p[1], p[2], n[0], n[1], n[2], u, v);
+  const D3 c = dual_sun_cosine(K, p[0], p[1], p[2], n[0], n[1], n[2], u, v);
   out[0] = c.v;
   out[1] = c.e;
+  out[2] = c.a;
 }
 
 // doCreatePrimaryRay (GPURendererKernel.java:573-724) for pixel (x, y) of a width x height frame with the

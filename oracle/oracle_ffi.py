@@ -168,6 +168,9 @@ class OracleDiff(ctypes.Structure):
         ("n_surfaces", ctypes.c_int64),
         ("albedo", ctypes.c_void_p),
         ("emission", ctypes.c_void_p),
+        ("n_checker", ctypes.c_int32),
+        ("surface_checker", ctypes.c_void_p),
+        ("checker", ctypes.c_void_p),
         ("turbidity", ctypes.c_double),
         ("elevation_deg", ctypes.c_double),
         ("azimuth_deg", ctypes.c_double),
@@ -175,26 +178,33 @@ class OracleDiff(ctypes.Structure):
 
 
 def loss_gradient(scene, params, sky, maps, albedo, emission, turbidity, elevation, azimuth, seeds, target, passes,
-                  crop, threads=None):
+                  crop, threads=None, checker=None):
     """FP64 forward-mode gradient of L = sum over the crop (x0, y0, w, h) of (film - target)^2, the film the mean
     of `passes` path-traced samples from `seeds` (the full frame's, width x height from the scene's camera) with
-    the differentiable parameters.  `sky` is the sky rendered with (Sky.set of turbidity, elevation, azimuth).
-    Returns (loss, gradient [albedo groups x RGB, emission, turbidity, elevation per degree], crop film)."""
+    the differentiable parameters.  `maps` is (surface_albedo, surface_emission) or, with checkerboard groups,
+    (surface_albedo, surface_emission, surface_checker) and `checker` their colours (groups x 2 x RGB).  `sky` is
+    the sky rendered with (Sky.set of turbidity, elevation, azimuth).  Returns (loss, gradient [albedo groups x
+    RGB, emission, checkerboard groups x 2 x RGB, turbidity, elevation per degree, azimuth per degree], crop
+    film)."""
     if threads is None:
         threads = min(16, os.cpu_count() or 1)
     width, height = int(scene.camera[16]), int(scene.camera[17])
     x0, y0, cw, ch = (int(v) for v in crop)
     sa = np.ascontiguousarray(maps[0], np.int32)
     se = np.ascontiguousarray(maps[1], np.int32)
+    sc = np.ascontiguousarray(maps[2], np.int32) if len(maps) > 2 else None
     al = np.ascontiguousarray(albedo, np.float32).reshape(-1)
     em = np.ascontiguousarray(emission, np.float32).reshape(-1)
+    ck = np.ascontiguousarray(checker if checker is not None else np.zeros(0), np.float32).reshape(-1)
+    nc = ck.size // 6 if sc is not None else 0
     d = OracleDiff(al.size // 3, em.size, sa.ctypes.data, se.ctypes.data, sa.size, al.ctypes.data, em.ctypes.data,
+                   nc, sc.ctypes.data if sc is not None else None, ck.ctypes.data,
                    float(turbidity), float(elevation), float(azimuth))
     sd = np.ascontiguousarray(seeds, np.int64)
     tg = np.ascontiguousarray(target, np.float32).reshape(-1)
     if sd.size != width * height or tg.size != width * height * 3:
         raise ValueError("seeds / target do not match the camera's %d x %d" % (width, height))
-    n = 3 * (al.size // 3) + em.size + 2
+    n = 3 * (al.size // 3) + em.size + 6 * nc + 3
     grad = np.zeros(n, np.float64)
     film = np.zeros(cw * ch * 3, np.float32)
     loss = ctypes.c_double()
@@ -210,31 +220,32 @@ def loss_gradient(scene, params, sky, maps, albedo, emission, turbidity, elevati
 
 
 def sky_dual(sky, turbidity, elevation, azimuth, direction):
-    """The FP64 adjoint's sky towards `direction`: (rgb, d rgb / d turbidity, d rgb / d elevation) or None
-    below the horizon."""
+    """The FP64 adjoint's sky towards `direction`: (rgb, d rgb / d turbidity, d rgb / d elevation, d rgb / d
+    azimuth) or None below the horizon."""
     d = (ctypes.c_float * 3)(*direction)
-    out = (ctypes.c_double * 9)()
+    out = (ctypes.c_double * 12)()
     f = lib().oracle_sky_dual
     f.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                   ctypes.c_void_p]
     if not f(ctypes.addressof(sky), turbidity, elevation, azimuth, ctypes.addressof(d), ctypes.addressof(out)):
         return None
     v = np.array(out[:], np.float64)
-    return v[0:3], v[3:6], v[6:9]
+    return v[0:3], v[3:6], v[6:9], v[9:12]
 
 
 def sun_cosine_dual(sky, turbidity, elevation, azimuth, point, normal, u, v):
-    """The FP64 adjoint's sun-sample cosine at random numbers (u, v): (cosine, d cosine / d elevation)."""
+    """The FP64 adjoint's sun-sample cosine at random numbers (u, v): (cosine, d cosine / d elevation,
+    d cosine / d azimuth)."""
     p = (ctypes.c_float * 3)(*point)
     n = (ctypes.c_float * 3)(*normal)
-    out = (ctypes.c_double * 2)()
+    out = (ctypes.c_double * 3)()
     f = lib().oracle_sun_cosine_dual
     f.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p,
                   ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_void_p]
     f.restype = None
     f(ctypes.addressof(sky), turbidity, elevation, azimuth, ctypes.addressof(p), ctypes.addressof(n), u, v,
       ctypes.addressof(out))
-    return out[0], out[1]
+    return out[0], out[1], out[2]
 
 
 def primary_ray(camera, renderer_type, width, height, x, y, seed):
@@ -301,7 +312,8 @@ def render_ray(scene, params, ray, seed, sky=None):
 PLANTS = {name: i for i, name in enumerate([
     "none", "camera_res", "camera_box", "lens_radius", "fisheye_w", "lambert_uniform", "phong_exponent",
     "fresnel_prob", "snell_ratio", "sun_cone", "ao_uniform", "phong_shader", "sphere_uv", "barycentric",
-    "normal_map_b", "noise_normal", "bullseye", "perez_theta0", "perez_swap", "fbm_texture", "clearcoat_offset"])}
+    "normal_map_b", "noise_normal", "bullseye", "perez_theta0", "perez_swap", "fbm_texture", "clearcoat_offset",
+    "checker_tile"])}
 
 
 class planted:

@@ -1,22 +1,21 @@
 """GPU: the differentiable path tracer and its path-replay adjoint (df_loss_gradient).
 
 Bars:
-* the differentiable forward at parameters equal to the texture values is the
-  ordinary render, bit for bit;
-* the adjoint gradient of L = sum (film - target)^2 with respect to albedo
-  groups, emission, turbidity and (sun off) sun elevation agrees with central
-  finite differences of the loss - same seeds for every render (common random
-  numbers), Russian roulette off - within 2 % (for gradient entries at least
-  1e-3 of the largest); the sun elevation with the sun's direct light on is
-  left out because moving the sun moves shadow boundaries, a visibility term
-  that path replay (interior term only) does not include;
-* on a 64 x 64 crop of the full 512 x 512 frame in exact math, the film is the
-  oracle's bit for bit and every gradient entry, the sun elevation with the
-  sun on included, agrees with the oracle's FP64 forward-mode gradient (dual
-  numbers through the sky and the sun sample, no finite differences) within
-  0.1 % (plus 1e-5 of the largest entry).
-The workload is the differentiation scene (347,680 triangles in 64 albedo
-groups, 4 area lights) rendered at 64 x 64, and at 512 x 512 for the crop.
+* the differentiable forward at parameters equal to the texture values (albedo groups, emissions and
+  the checkerboard ground's two colours) is the ordinary render, bit for bit;
+* the adjoint gradient of L = sum (film - target)^2 with respect to albedo groups, emission,
+  checkerboard colours, turbidity and (sun off) the sun's elevation and azimuth agrees with central
+  finite differences of the loss - same seeds for every render (common random numbers), Russian
+  roulette off - within 2 % (for gradient entries at least 1e-3 of the largest); the sun angles with
+  the sun's direct light on are left out because moving the sun moves shadow boundaries, a visibility
+  term that path replay (interior term only) does not include;
+* on a 64 x 64 crop of the full 512 x 512 frame in exact math, the film is the oracle's bit for bit and
+  every gradient entry, the sun angles with the sun on included, agrees with the oracle's FP64
+  forward-mode gradient (dual numbers through the sky and the sun sample on both sides) within 0.1 %
+  (plus 1e-5 of the largest entry);
+* the torch binding's gradients are the C ABI's; misuse is refused.
+The workload is the differentiation scene (347,680 triangles in 64 albedo groups, 4 area lights, a
+checkerboard ground) rendered at 64 x 64, and at 512 x 512 for the crop.
 """
 
 import numpy as np
@@ -53,18 +52,30 @@ def _kernel(sc, w, h):
     return s, k
 
 
-def _params(n_albedo, n_emission, seed):
-    """C5's parameter distribution: albedo in [0.1, 0.9], emission in [1, 20], turbidity in [2, 6],
-    sun elevation in [10, 70] degrees."""
+def _maps(s):
+    sa, se, na, ne = dfa.diff_maps(s)
+    sc, nc, _ = dfa.diff_checker_map(s)
+    return (sa, se, sc), na, ne, nc
+
+
+def _params(n_albedo, n_emission, seed, n_checker=1):
+    """The differentiation workload's parameter distribution: albedo and checkerboard colours in [0.1, 0.9],
+    emission in [1, 20], turbidity in [2, 6], sun elevation in [10, 70] and azimuth in [0, 90] degrees."""
     r = np.random.default_rng(seed)
     return {"albedo": r.uniform(0.1, 0.9, (n_albedo, 3)).astype(np.float32),
             "emission": r.uniform(1.0, 20.0, n_emission).astype(np.float32),
-            "turbidity": float(r.uniform(2.0, 6.0)), "elevation": float(r.uniform(10.0, 70.0))}
+            "turbidity": float(r.uniform(2.0, 6.0)), "elevation": float(r.uniform(10.0, 70.0)),
+            "checker": r.uniform(0.1, 0.9, (n_checker, 2, 3)).astype(np.float32),
+            "azimuth": float(r.uniform(0.0, 90.0))}
+
+
+def _set(k, maps, th, sun=True):
+    k.setDifferentiable(maps[0], maps[1], th["albedo"], th["emission"], th["turbidity"], th["elevation"],
+                        th.get("azimuth", AZIMUTH), sun_active=sun, surface_checker=maps[2], checker=th["checker"])
 
 
 def _loss(k, maps, th, target, passes, grad=False, sun=True, seed=SEED):
-    k.setDifferentiable(maps[0], maps[1], th["albedo"], th["emission"], th["turbidity"], th["elevation"], AZIMUTH,
-                        sun_active=sun)
+    _set(k, maps, th, sun)
     k.seed(seed)
     return k.lossGradient(target, passes, gradient=grad)
 
@@ -78,12 +89,13 @@ def test_diff_forward_at_texture_values_is_the_plain_render(gpu, diff_scene):
     w, h, passes = 48, 40, 3
     s, k = _kernel(diff_scene, w, h)
     try:
-        sa, se, na, ne = dfa.diff_maps(s)
+        maps, na, ne, _ = _maps(s)
+        sa, se, _ = maps
         recip = np.float32(1.0) / np.float32(255.0)
         albedo = np.zeros((na, 3), np.float32)
         emission = np.zeros(ne, np.float32)
         for rec in range(s.surfaces.size // 6):
-            for slot, field, out in ((sa[rec], 1, None), (se[rec], 2, None)):
+            for slot, field in ((sa[rec], 1), (se[rec], 2)):
                 if slot < 0:
                     continue
                 off = int(s.surfaces[rec * 6 + field])
@@ -95,10 +107,15 @@ def test_diff_forward_at_texture_values_is_the_plain_render(gpu, diff_scene):
                 else:
                     assert rgb[0] == rgb[1] == rgb[2]
                     emission[slot] = rgb[0]
-        th = {"albedo": albedo, "emission": emission, "turbidity": 3.0, "elevation": 45.0}
-        _loss(k, (sa, se), th, np.zeros(w * h * 3, np.float32), passes)
+        th = {"albedo": albedo, "emission": emission, "turbidity": 3.0, "elevation": 45.0, "azimuth": AZIMUTH,
+              "checker": dfa.diff_checker_map(s)[2]}
+        _loss(k, maps, th, np.zeros(w * h * 3, np.float32), passes)
         film = k.getFilm()[0].copy()
         seeds = k.getSeeds()
+        # the checkerboard colours swapped: the ground's pixels change, so the check above is not vacuous
+        th2 = dict(th, checker=th["checker"][:, ::-1].copy())
+        _loss(k, maps, th2, np.zeros(w * h * 3, np.float32), passes)
+        swapped = k.getFilm()[0].copy()
     finally:
         k.dispose()
     plain = s.with_camera(s.camera)
@@ -117,6 +134,7 @@ def test_diff_forward_at_texture_values_is_the_plain_render(gpu, diff_scene):
         np.testing.assert_array_equal(seeds, k2.getSeeds())
     finally:
         k2.dispose()
+    assert not np.array_equal(swapped, film)
 
 
 def _fd(k, maps, th, target, passes, key, index, hstep, sun=True):
@@ -145,12 +163,12 @@ def test_adjoint_matches_central_differences(gpu, diff_scene):
     passes = 4
     s, k = _kernel(diff_scene, w, h)
     try:
-        sa, se, na, ne = dfa.diff_maps(s)
-        maps = (sa, se)
+        maps, na, ne, nc = _maps(s)
         target = _target(k, maps, _params(na, ne, 500), w, h)
         th = _params(na, ne, 501)
         loss, g, fms, ams = _loss(k, maps, th, target, passes, grad=True)
         assert loss > 0.0 and fms > 0.0 and ams > 0.0
+        assert g.size == 3 * na + ne + 6 * nc + 3
         gmax = float(np.max(np.abs(g)))
         ga = g[:3 * na]
         # the albedo entries with the largest gradients, plus a few small ones
@@ -161,35 +179,39 @@ def test_adjoint_matches_central_differences(gpu, diff_scene):
         for j in range(ne):
             _check(g[3 * na + j], _fd(k, maps, th, target, passes, "emission", j, 1e-3 * float(th["emission"][j])),
                    gmax, "emission %d" % j)
-        _check(g[3 * na + ne], _fd(k, maps, th, target, passes, "turbidity", None, 1e-2), gmax, "turbidity")
+        gc = g[3 * na + ne:3 * na + ne + 6 * nc]
+        assert np.count_nonzero(np.abs(gc) > 1e-3 * gmax) >= 3  # the ground is in view
+        for i in range(6 * nc):
+            _check(gc[i], _fd(k, maps, th, target, passes, "checker", i, 2e-3), gmax, "checker %d" % i)
+        _check(g[-3], _fd(k, maps, th, target, passes, "turbidity", None, 1e-2), gmax, "turbidity")
     finally:
         k.dispose()
 
 
-def test_adjoint_sun_elevation_through_the_sky(gpu, diff_scene):
-    """Sun direct light off: the elevation acts through the sky model only (smooth in the parameter)."""
+def test_adjoint_sun_angles_through_the_sky(gpu, diff_scene):
+    """Sun direct light off: the elevation and azimuth act through the sky model only (smooth in the
+    parameters)."""
     w = h = 64
     passes = 4
     s, k = _kernel(diff_scene, w, h)
     try:
-        sa, se, na, ne = dfa.diff_maps(s)
-        maps = (sa, se)
+        maps, na, ne, _ = _maps(s)
         target = _target(k, maps, _params(na, ne, 500), w, h, sun=False)
         th = _params(na, ne, 502)
         loss, g, _, _ = _loss(k, maps, th, target, passes, grad=True, sun=False)
         gmax = float(np.max(np.abs(g)))
-        fd = _fd(k, maps, th, target, passes, "elevation", None, 0.1, sun=False)
-        _check(g[-1], fd, gmax, "elevation")
-        assert abs(fd[0]) > 1e-3 * gmax  # the check is not vacuous
-        _check(g[3 * na + ne], _fd(k, maps, th, target, passes, "turbidity", None, 1e-2, sun=False), gmax,
-               "turbidity")
+        for key, idx, step in (("elevation", -2, 0.1), ("azimuth", -1, 0.2)):
+            fd = _fd(k, maps, th, target, passes, key, None, step, sun=False)
+            _check(g[idx], fd, gmax, key)
+            assert abs(fd[0]) > 1e-4 * gmax, key  # the check is not vacuous
+        _check(g[-3], _fd(k, maps, th, target, passes, "turbidity", None, 1e-2, sun=False), gmax, "turbidity")
     finally:
         k.dispose()
 
 
 def test_adjoint_matches_the_fp64_cpu_adjoint_on_a_crop(gpu, diff_scene):
     w = h = 512
-    x0, y0, cw, ch = crop = (160, 120, 64, 64)
+    x0, y0, cw, ch = crop = (160, 300, 64, 64)  # the sphere's lower edge over the checkerboard ground
     passes = 4
     s, k = _kernel(diff_scene, w, h)
     try:
@@ -197,8 +219,7 @@ def test_adjoint_matches_the_fp64_cpu_adjoint_on_a_crop(gpu, diff_scene):
         p.math_mode = dfa.MATH_EXACT
         p.pt_engine = dfa.PT_MEGAKERNEL
         k.setParams(p)
-        sa, se, na, ne = dfa.diff_maps(s)
-        maps = (sa, se)
+        maps, na, ne, nc = _maps(s)
         other = _target(k, maps, _params(na, ne, 500), w, h)
         th = _params(na, ne, 501)
         # the target is the render at th outside the crop, so only the crop carries loss
@@ -207,7 +228,7 @@ def test_adjoint_matches_the_fp64_cpu_adjoint_on_a_crop(gpu, diff_scene):
         target = film_th.copy()
         view = target.reshape(h, w, 3)
         view[y0:y0 + ch, x0:x0 + cw] = other.reshape(h, w, 3)[y0:y0 + ch, x0:x0 + cw]
-        k.setDifferentiable(maps[0], maps[1], th["albedo"], th["emission"], th["turbidity"], th["elevation"], AZIMUTH)
+        _set(k, maps, th)
         k.seed(SEED)
         seeds = k.getSeeds().copy()
         loss, g, _, _ = k.lossGradient(target, passes, gradient=True)
@@ -215,13 +236,16 @@ def test_adjoint_matches_the_fp64_cpu_adjoint_on_a_crop(gpu, diff_scene):
         np.testing.assert_array_equal(film, film_th)
     finally:
         k.dispose()
-    sky = dfa.sky_at(th["elevation"], AZIMUTH, th["turbidity"])
+    sky = dfa.sky_at(th["elevation"], th["azimuth"], th["turbidity"])
     loss_o, g_o, film_o = oracle_ffi.loss_gradient(s, p, sky, maps, th["albedo"], th["emission"], th["turbidity"],
-                                                   th["elevation"], AZIMUTH, seeds, target, passes, crop)
+                                                   th["elevation"], th["azimuth"], seeds, target, passes, crop,
+                                                   checker=th["checker"])
     np.testing.assert_array_equal(film.reshape(h, w, 3)[y0:y0 + ch, x0:x0 + cw].reshape(-1), film_o)
     assert abs(loss - loss_o) <= 1e-6 * loss_o  # the GPU squares float differences, the oracle double ones
     gmax = float(np.max(np.abs(g_o)))
-    assert np.count_nonzero(g_o[:3 * na]) > 10 and g_o[-2] != 0.0 and g_o[-1] != 0.0
+    gc = g_o[3 * na + ne:3 * na + ne + 6 * nc]
+    assert np.count_nonzero(g_o[:3 * na]) > 10 and np.count_nonzero(gc) == 6
+    assert g_o[-3] != 0.0 and g_o[-2] != 0.0 and g_o[-1] != 0.0
     bad = np.abs(g - g_o) > 1e-3 * np.abs(g_o) + 1e-5 * gmax
     assert not bad.any(), [(int(i), g[i], g_o[i]) for i in np.flatnonzero(bad)]
 
@@ -233,21 +257,24 @@ def test_torch_autograd_binding(gpu, diff_scene):
     w = h = 32
     s, k = _kernel(diff_scene, w, h)
     try:
-        sa, se, na, ne = dfa.diff_maps(s)
-        maps = (sa, se)
+        maps, na, ne, nc = _maps(s)
         target = _target(k, maps, _params(na, ne, 500), w, h)
         th = _params(na, ne, 503)
         albedo = torch.tensor(th["albedo"], dtype=torch.float64, requires_grad=True)
         emission = torch.tensor(th["emission"], dtype=torch.float64, requires_grad=True)
+        checker = torch.tensor(th["checker"], dtype=torch.float64, requires_grad=True)
         turb = torch.tensor(th["turbidity"], dtype=torch.float64, requires_grad=True)
         elev = torch.tensor(th["elevation"], dtype=torch.float64, requires_grad=True)
-        loss = render_loss(k, maps, target, 2, SEED, albedo, emission, turb, elev, AZIMUTH)
+        azim = torch.tensor(th["azimuth"], dtype=torch.float64, requires_grad=True)
+        loss = render_loss(k, maps, target, 2, SEED, albedo, emission, turb, elev, azim, checker=checker)
         (3.0 * loss).backward()
         _, g, _, _ = _loss(k, maps, th, target, 2, grad=True)
         np.testing.assert_allclose(albedo.grad.numpy().reshape(-1), 3.0 * g[:3 * na], rtol=1e-6)
         np.testing.assert_allclose(emission.grad.numpy(), 3.0 * g[3 * na:3 * na + ne], rtol=1e-6)
-        assert abs(turb.grad.item() - 3.0 * g[-2]) <= 1e-6 * abs(g[-2]) + 1e-30
-        assert abs(elev.grad.item() - 3.0 * g[-1]) <= 1e-6 * abs(g[-1]) + 1e-30
+        np.testing.assert_allclose(checker.grad.numpy().reshape(-1), 3.0 * g[3 * na + ne:3 * na + ne + 6 * nc],
+                                   rtol=1e-6)
+        for t, i in ((turb, -3), (elev, -2), (azim, -1)):
+            assert abs(t.grad.item() - 3.0 * g[i]) <= 1e-6 * abs(g[i]) + 1e-30
     finally:
         k.dispose()
 
@@ -255,11 +282,12 @@ def test_torch_autograd_binding(gpu, diff_scene):
 def test_adjoint_argument_checks(gpu, diff_scene):
     s, k = _kernel(diff_scene, 16, 16)
     try:
-        sa, se, na, ne = dfa.diff_maps(s)
+        maps, na, ne, nc = _maps(s)
+        sa, se, sc = maps
         th = _params(na, ne, 1)
         with pytest.raises(dfa.HIPError):  # not in differentiable mode yet
             k.lossGradient(np.zeros(16 * 16 * 3, np.float32), 1)
-        k.setDifferentiable(sa, se, th["albedo"], th["emission"], th["turbidity"], th["elevation"], AZIMUTH)
+        _set(k, maps, th)
         with pytest.raises(dfa.HIPError):
             k.lossGradient(np.zeros(10, np.float32), 1)
         p = s.copy_params()
@@ -270,7 +298,19 @@ def test_adjoint_argument_checks(gpu, diff_scene):
             k.lossGradient(np.zeros(16 * 16 * 3, np.float32), 1)
         loss, g, _, _ = k.lossGradient(np.zeros(16 * 16 * 3, np.float32), 1, gradient=False)
         assert g is None and loss > 0.0
+        k.setParams(s.copy_params())
         with pytest.raises(dfa.HIPError):  # surface map of the wrong length
             k.setDifferentiable(sa[:-1], se[:-1], th["albedo"], th["emission"], 3.0, 45.0, AZIMUTH)
+        bad = sc.copy()
+        bad[int(np.flatnonzero(sa >= 0)[0])] = 0  # a checkerboard group on a surface with an albedo group
+        with pytest.raises(dfa.HIPError):
+            k.setDifferentiable(sa, se, th["albedo"], th["emission"], 3.0, 45.0, AZIMUTH, surface_checker=bad,
+                                checker=th["checker"])
+        # the sky must stay the one df_set_diff was given
+        _set(k, maps, th)
+        k.lossGradient(np.zeros(16 * 16 * 3, np.float32), 1)
+        k.updateSunAndSky(dfa.sky_at(th["elevation"], th["azimuth"] + 10.0, th["turbidity"]))
+        with pytest.raises(dfa.HIPError):
+            k.lossGradient(np.zeros(16 * 16 * 3, np.float32), 1)
     finally:
         k.dispose()
