@@ -176,11 +176,26 @@ def test_fast_mode_ray_cast_and_ao_buffers(scenes, gpu, cid, w, h, renderer):
     g = _gpu_render(sc, p, 1)
     o = _oracle_render(sc, p, 1, g["seeds0"])
     np.testing.assert_array_equal(g["prims"], o.primitive_offsets)
-    film = g["film"][0]
-    rel = np.abs(film - o.film) / np.maximum(np.abs(o.film), 1e-3)
-    # AO hemisphere directions use cos/sin: a ray that grazes an occluder may flip
-    bad = np.mean(rel > 1e-4)
-    assert bad <= (0.002 if renderer == 1 else 0.0), bad
+    _check_fast_buffer(sc, g["film"][0], o, renderer)
+
+
+def _check_fast_buffer(sc, film, o, renderer):
+    """Fast-mode ray-cast, AO and normal buffers agree with the oracle within 1e-4 relative on every pixel,
+    with one exception for AO: a pixel whose primary ray hits the terrain may flip.  The terrain normal is
+    a difference of sine products 0.04 apart (GPURendererKernel.java:2495-2539); a last-ulp difference of
+    the float sine is amplified ~25x by that cancellation and tilts the normal, and with it the pixel's one
+    AO ray, which can then pass on the other side of an occluder's silhouette.  Such pixels are at most
+    2e-5 of the frame (measured: 7 of 2,073,600 at 1920 x 1080, none at the small sizes)."""
+    rel = (np.abs(film - o.film) / np.maximum(np.abs(o.film), 1e-3)).reshape(-1, 3).max(axis=1)
+    bad = np.flatnonzero(rel > 1e-4)
+    if renderer != 1:
+        assert bad.size == 0, bad[:10]
+        return
+    types = sc.primitives.reshape(-1, 3)[:, 0]
+    on_terrain = np.array([o.primitive_offsets[i] >= 0 and types[o.primitive_offsets[i] // 3] == 3 for i in bad],
+                          bool)
+    assert on_terrain.all(), bad[~on_terrain][:10]
+    assert bad.size <= max(1, 2e-5 * rel.size), bad.size
 
 
 @pytest.mark.parametrize("cid,w,h", [(1, 64, 64), (3, 96, 54)])
@@ -243,6 +258,18 @@ def test_fast_mode_same_stream_tone_mapped_bytes(scenes, gpu):
     o = _oracle_render(sc, p, 16, g["seeds0"])
     d = np.abs(g["bgra"].astype(int) - o.image_bgra.astype(int)).reshape(-1, 4)[:, :3].max(axis=1)
     assert np.mean(d <= 1) > 0.97, np.mean(d <= 1)
+
+
+def test_full_size_fast_ambient_occlusion(scenes, gpu):
+    """The 1920 x 1080 mesh + terrain workload's AO buffer in fast math against the oracle on every pixel."""
+    sc = scenes(3)
+    p = sc.copy_params()
+    p.renderer_type = 1
+    p.math_mode = dfa.MATH_FAST
+    g = _gpu_render(sc, p, 1)
+    o = _oracle_render(sc, p, 1, g["seeds0"])
+    np.testing.assert_array_equal(g["prims"], o.primitive_offsets)
+    _check_fast_buffer(sc, g["film"][0], o, 1)
 
 
 @pytest.mark.parametrize("cid", [2, 3])
