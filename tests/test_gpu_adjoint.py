@@ -8,7 +8,9 @@ Bars:
   finite differences of the loss - same seeds for every render (common random numbers), Russian
   roulette off - within 2 % (for gradient entries at least 1e-3 of the largest); the sun angles with
   the sun's direct light on are left out because moving the sun moves shadow boundaries, a visibility
-  term that path replay (interior term only) does not include;
+  term that path replay (interior term only) does not include; at 64 x 64 x 4 samples for every kind of
+  entry, and at the full workload (512 x 512 x 32 samples) for the largest entries of each kind, where the
+  adjoint must also take at most 3x the forward's time;
 * on a 64 x 64 crop of the full 512 x 512 frame in exact math, the film is the oracle's bit for bit and
   every gradient entry, the sun angles with the sun on included, agrees with the oracle's FP64
   forward-mode gradient (dual numbers through the sky and the sun sample on both sides) within 0.1 %
@@ -205,6 +207,41 @@ def test_adjoint_sun_angles_through_the_sky(gpu, diff_scene):
             _check(g[idx], fd, gmax, key)
             assert abs(fd[0]) > 1e-4 * gmax, key  # the check is not vacuous
         _check(g[-3], _fd(k, maps, th, target, passes, "turbidity", None, 1e-2, sun=False), gmax, "turbidity")
+    finally:
+        k.dispose()
+
+
+def test_adjoint_at_the_full_differentiation_workload(gpu, diff_scene):
+    """512 x 512 at 32 samples per pixel, as the benchmark runs it: the largest albedo, emission and
+    checkerboard entries and turbidity against central differences, with the sun on; then the sun angles
+    with the sun's direct light off."""
+    w = h = 512
+    passes = 32
+    s, k = _kernel(diff_scene, w, h)
+    try:
+        maps, na, ne, nc = _maps(s)
+        for sun in (True, False):
+            target = _target(k, maps, _params(na, ne, 500), w, h, passes=passes, sun=sun)
+            th = _params(na, ne, 501)
+            loss, g, fms, ams = _loss(k, maps, th, target, passes, grad=True, sun=sun)
+            assert ams <= 3.0 * fms, (ams, fms)
+            gmax = float(np.max(np.abs(g)))
+            if sun:
+                ga = g[:3 * na]
+                for i in np.argsort(-np.abs(ga))[:3]:
+                    _check(ga[i], _fd(k, maps, th, target, passes, "albedo", int(i), 1e-3), gmax, "albedo %d" % i)
+                j = int(np.argmax(np.abs(g[3 * na:3 * na + ne])))
+                _check(g[3 * na + j], _fd(k, maps, th, target, passes, "emission", j, 1e-3 * float(th["emission"][j])),
+                       gmax, "emission %d" % j)
+                gc = g[3 * na + ne:3 * na + ne + 6 * nc]
+                for i in np.argsort(-np.abs(gc))[:2]:
+                    _check(gc[i], _fd(k, maps, th, target, passes, "checker", int(i), 2e-3), gmax, "checker %d" % i)
+                _check(g[-3], _fd(k, maps, th, target, passes, "turbidity", None, 1e-2), gmax, "turbidity")
+            else:
+                for key, idx, step in (("elevation", -2, 0.1), ("azimuth", -1, 0.2)):
+                    fd = _fd(k, maps, th, target, passes, key, None, step, sun=False)
+                    _check(g[idx], fd, gmax, key)
+                    assert abs(fd[0]) > 1e-4 * gmax, key
     finally:
         k.dispose()
 
