@@ -233,6 +233,19 @@ struct Tracer {
 
   // The primitive loop of doIntersectPrimitives: the closest (or, testing,
   // the first) hit's distance, primitive index, shape type and shape offset.
+  //
+  // An any-hit query (testing) keeps `closest` at the no-hit value until it
+  // ends, so no test in it depends on what was tested before: each shape's
+  // test compares with `closest` only to reject (the box cull's
+  // entry-or-exit-before-closest rule then reduces to "entry or exit
+  // ahead"), and the query ends at the first hit.  Whether some primitive is
+  // hit is therefore the same in any order, and that is all its callers read
+  // (the returned distance is only compared with the no-hit value).  So an
+  // any-hit query tests a block's primitives other than meshes first, in
+  // index order, and walks its mesh runs after them: a ray blocked by the
+  // terrain, a sphere, a plane or a single triangle skips the BVH walks
+  // (trace 9.93 -> 9.25 ms per 1080p pass, box tests -27 %, the film's bits
+  // unchanged).  A closest-hit query runs the reference's index order.
   template <bool testing>
   __device__ __forceinline__ float query(float ox, float oy, float oz, float dx, float dy, float dz, int& ci,
                                          int& cst, int& cso) {
@@ -245,7 +258,8 @@ struct Tracer {
     // Primitives in blocks of 64: first a branch-free pass over the block's
     // bounding spheres marks, per lane, the primitives the ray can reach
     // (approximate reciprocal and square root: the spheres' enlargement
-    // covers their error), then the marked primitives run in index order.
+    // covers their error), then the marked primitives run (in index order,
+    // or for an any-hit query in the two phases described above).
     const float dd = dx * dx + dy * dy + dz * dz;
     const float ia = __builtin_amdgcn_rcpf(dd), dl = __builtin_amdgcn_sqrtf(dd);
     for (int base = 0; base < n; base += 64) {
@@ -268,55 +282,69 @@ struct Tracer {
           any |= __ballot(keep) != 0 ? uint64_t{1} << j : uint64_t{0};
         }
       }
-      while (any) {
-        const int j = __builtin_ctzll(any);
-        const DPrim pr = load_uniform(S.prims, base + j);
-        if (pr.type == 5) {
-          // A run of consecutive meshes (a single mesh is a run of one): each
-          // lane walks its own marked meshes of the run, in index order, in one
-          // loop whose steps are single node visits, so a lane whose walk ends
-          // moves straight on to its next mesh instead of waiting for the wave.
-          // Every lane's own sequence of tests, and so every result, is the
-          // reference's.
-          const int end = (pr.run_end < base + m ? pr.run_end : base + m) - base;
-          const uint64_t run = (end >= 64 ? ~uint64_t{0} : ((uint64_t{1} << end) - 1)) & ~((uint64_t{1} << j) - 1);
-          any &= ~run;
-          if (walk_meshes(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing)) return closest;
-          continue;
+      // testing: phase 0 the primitives other than meshes, phase 1 the mesh runs
+      const uint64_t marked = any;
+      for (int phase = testing ? 0 : 1; phase < 2; ++phase) {
+        any = marked;
+        while (any) {
+          const int j = __builtin_ctzll(any);
+          const DPrim pr = load_uniform(S.prims, base + j);
+          if (testing && phase == 0 && pr.type == 5) {  // a mesh run: left for phase 1
+            const int end = (pr.run_end < base + m ? pr.run_end : base + m) - base;
+            any &= ~((end >= 64 ? ~uint64_t{0} : ((uint64_t{1} << end) - 1)) & ~((uint64_t{1} << j) - 1));
+            continue;
+          }
+          if (testing && phase == 1 && pr.type != 5) {  // tested in phase 0
+            any &= any - 1;
+            continue;
+          }
+          if (pr.type == 5) {
+            // A run of consecutive meshes (a single mesh is a run of one): each
+            // lane walks its own marked meshes of the run, in index order, in one
+            // loop whose steps are single node visits, so a lane whose walk ends
+            // moves straight on to its next mesh instead of waiting for the wave.
+            // Every lane's own sequence of tests, and so every result, is the
+            // reference's.
+            const int end = (pr.run_end < base + m ? pr.run_end : base + m) - base;
+            const uint64_t run = (end >= 64 ? ~uint64_t{0} : ((uint64_t{1} << end) - 1)) & ~((uint64_t{1} << j) - 1);
+            any &= ~run;
+            if (walk_meshes(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing)) return closest;
+            continue;
+          }
+          any &= any - 1;
+          if (!((cand >> j) & 1)) continue;
+          const int i = base + j;
+          const int st = pr.type;
+          ++prim_tests;
+          const float* m = pr.w2o;
+          const float oxo = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
+          const float oyo = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
+          const float ozo = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
+          const float dxo = m[0] * dx + m[1] * dy + m[2] * dz;
+          const float dyo = m[4] * dx + m[5] * dy + m[6] * dz;
+          const float dzo = m[8] * dx + m[9] * dy + m[10] * dz;
+          float cur = DF_INF;
+          if (st == 1) {
+            cur = isect_plane(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], pr.geo[4],
+                              pr.geo[5], closest);
+          } else if (st == 2) {
+            ++sphere_tests;
+            cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
+          } else if (st == 3) {
+            cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n, &terrain_steps);
+          } else if (st == 4) {
+            const DTri tr = S.tris[pr.node_root];
+            ++tri_tests;
+            cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
+          }
+          if (cur < closest) {
+            closest = cur;
+            ci = i;
+            cst = st;
+            cso = pr.shape_offset;
+          }
+          if (testing && ci != -1) return closest;
         }
-        any &= any - 1;
-        if (!((cand >> j) & 1)) continue;
-        const int i = base + j;
-        const int st = pr.type;
-        ++prim_tests;
-        const float* m = pr.w2o;
-        const float oxo = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
-        const float oyo = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
-        const float ozo = m[8] * ox + m[9] * oy + m[10] * oz + m[11];
-        const float dxo = m[0] * dx + m[1] * dy + m[2] * dz;
-        const float dyo = m[4] * dx + m[5] * dy + m[6] * dz;
-        const float dzo = m[8] * dx + m[9] * dy + m[10] * dz;
-        float cur = DF_INF;
-        if (st == 1) {
-          cur = isect_plane(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], pr.geo[4],
-                            pr.geo[5], closest);
-        } else if (st == 2) {
-          ++sphere_tests;
-          cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
-        } else if (st == 3) {
-          cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n, &terrain_steps);
-        } else if (st == 4) {
-          const DTri tr = S.tris[pr.node_root];
-          ++tri_tests;
-          cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
-        }
-        if (cur < closest) {
-          closest = cur;
-          ci = i;
-          cst = st;
-          cso = pr.shape_offset;
-        }
-        if (testing && ci != -1) return closest;
       }
     }
     return closest;
