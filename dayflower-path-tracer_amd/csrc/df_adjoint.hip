@@ -184,11 +184,11 @@ __device__ void replay_path(Tracer<M>& tr, double* g, const DAdjoint& A, const f
 // unconstrained allocation allows (and than four).
 template <int M>
 __global__ __launch_bounds__(256, 3) void adjoint_kernel(DScene S, DCamera C, DSky K, DParams P, DFrame F, DAdjoint A) {
-  __shared__ int perm[512];
+  __shared__ int perm[kPermLen];
   __shared__ double g[kMaxDiffParams];
   const int tid = threadIdx.y * 16 + threadIdx.x;
   const int n_params = sky_row(A) + 3;
-  for (int i = tid; i < 512; i += 256) perm[i] = kPerm[i & 255];
+  load_perm(perm, tid, 256);
   for (int i = tid; i < n_params; i += 256) g[i] = 0.0;
   __syncthreads();
   const int x = blockIdx.x * 16 + threadIdx.x;

@@ -144,8 +144,11 @@ struct Rng {
 // The permutation table (Ken Perlin's improved-noise table, as tabulated in
 // AbstractKernel.java:1130) is staged in LDS by each block; gradient indices
 // are perm % 12 and the 12 gradients of grad3 are computed from the index.
+// Entries of the noise lookup table: the permutation twice, then the same modulo 12.
+constexpr int kPermLen = 1024;
+
 struct Noise {
-  const int* perm;  // 512 entries in LDS
+  const int* perm;  // kPermLen entries (in LDS where noise is evaluated)
 
   static __device__ __forceinline__ float grad_dot3(int g, float x, float y, float z) {
     // simplexGradient3 rows: (1,1,0) (-1,1,0) (1,-1,0) (-1,-1,0) (1,0,1) (-1,0,1)
@@ -185,7 +188,7 @@ struct Noise {
     const int i = jf2i(v);
     return v < static_cast<float>(i) ? i - 1 : i;
   }
-  __device__ __forceinline__ int p12(int i) const { return perm[i] % 12; }
+  __device__ __forceinline__ int p12(int i) const { return perm[512 + i]; }  // perm[i] % 12
 
   __device__ float simplex2(float x, float y) const {
     const float F2 = 0.3660254037844386f, G2 = 0.21132486540518713f;

@@ -83,7 +83,7 @@ __device__ __forceinline__ void render_pixel(const DScene& S, const DCamera& C, 
 }
 
 __device__ __forceinline__ void stage_perm_2d(int* perm) {
-  for (int i = threadIdx.y * 16 + threadIdx.x; i < 512; i += 256) perm[i] = kPerm[i & 255];
+  load_perm(perm, threadIdx.y * 16 + threadIdx.x, 256);
   __syncthreads();
 }
 
@@ -97,7 +97,7 @@ __device__ __forceinline__ void stage_perm_2d(int* perm) {
 // tile gets one work-item and the same arithmetic.
 template <int M, int R>
 __global__ __launch_bounds__(256, 4) void render_kernel(DScene S, DCamera C, DSky K, DParams P, DFrame F) {
-  __shared__ int perm[512];
+  __shared__ int perm[kPermLen];
   stage_perm_2d(perm);
   int tx = threadIdx.x, ty = threadIdx.y;
   if (P.wave_tile == 1) {
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256, 4) void render_kernel(DScene S, DCamera C, DSk
 template <int M, int R>
 __global__ __launch_bounds__(256, 6) void batch_kernel(const DJob* __restrict__ jobs,
                                                     const DBatchItem* __restrict__ items, int n_items, int* head) {
-  __shared__ int perm[512];
+  __shared__ int perm[kPermLen];
   __shared__ int next;
   stage_perm_2d(perm);
   for (;;) {

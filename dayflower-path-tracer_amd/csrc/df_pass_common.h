@@ -9,7 +9,7 @@
 namespace dfk {
 
 // Ken Perlin's improved-noise permutation (as tabulated in AbstractKernel.java:1130).
-static __constant__ int kPerm[256] = {
+static constexpr int kPerm[256] = {
     151, 160, 137, 91,  90,  15,  131, 13,  201, 95,  96,  53,  194, 233, 7,   225, 140, 36,  103, 30,  69,  142,
     8,   99,  37,  240, 21,  10,  23,  190, 6,   148, 247, 120, 234, 75,  0,   26,  197, 62,  94,  252, 219, 203,
     117, 35,  11,  32,  57,  177, 33,  88,  237, 149, 56,  87,  174, 20,  125, 136, 171, 168, 68,  175, 74,  165,
@@ -22,6 +22,29 @@ static __constant__ int kPerm[256] = {
     242, 193, 238, 210, 144, 12,  191, 179, 162, 241, 81,  51,  145, 235, 249, 14,  239, 107, 49,  192, 214, 31,
     181, 199, 106, 157, 184, 84,  204, 176, 115, 121, 50,  45,  127, 4,   150, 254, 138, 236, 205, 93,  222, 114,
     67,  29,  24,  72,  243, 141, 128, 195, 78,  66,  215, 61,  156, 180};
+
+// The noise lookup table: the permutation twice (512 entries, as the
+// reference's noisePermutations), then each entry modulo 12 (512 more, as its
+// noisePermutationsModulo12, AbstractKernel.java:1126-1141): simplex noise
+// reads its gradient indices from the second half instead of computing
+// `perm[i] % 12`.
+struct PermTable {
+  int v[kPermLen];
+};
+constexpr PermTable make_perm_table() {
+  PermTable t{};
+  for (int i = 0; i < 512; ++i) {
+    t.v[i] = kPerm[i & 255];
+    t.v[512 + i] = kPerm[i & 255] % 12;
+  }
+  return t;
+}
+static __constant__ PermTable kPermTable = make_perm_table();
+
+// Copies the table into the block's LDS copy (every thread of the block calls it).
+__device__ __forceinline__ void load_perm(int* perm, int tid, int nthreads) {
+  for (int i = tid; i < kPermLen; i += nthreads) perm[i] = kPermTable.v[i];
+}
 
 // Tone map + sRGB + BGRA (run() :282-307, AbstractImageKernel.java:410-472, 503-774).
 template <int M>

@@ -43,7 +43,7 @@ constexpr int32_t kFinish = 1 << 30;  // shadow-queue flag: the path ends once r
 constexpr int kThreads = 256;
 
 __device__ __forceinline__ void stage_perm(int* perm) {
-  for (int i = threadIdx.x + threadIdx.y * blockDim.x; i < 512; i += blockDim.x * blockDim.y) perm[i] = kPerm[i & 255];
+  load_perm(perm, threadIdx.x + threadIdx.y * blockDim.x, blockDim.x * blockDim.y);
   __syncthreads();
 }
 
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
   int gid = 0;
   if (x < P.width && y < P.height) {
     gid = y * P.width + x;
-    Tracer<M> tr(S, C, K, P, kPerm);
+    Tracer<M> tr(S, C, K, P, kPermTable.v);
     tr.rng.s = F.seeds[gid];
     float ray[6];
     valid = tr.primary_ray(x, y, ray);
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
                                                      int bounce) {
-  __shared__ int perm[512];
+  __shared__ int perm[kPermLen];
   stage_perm(perm);
   Tracer<M> tr(S, C, K, P, perm);
   tr.noise_in = W.noise;  // computed by wf_normals
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
 template <int M>
 __global__ __launch_bounds__(kThreads) void wf_normals(DScene S, DCamera C, DSky K, DParams P, DPaths W,
                                                        const int32_t* count, int32_t* head) {
-  __shared__ int perm[512];
+  __shared__ int perm[kPermLen];
   stage_perm(perm);
   Tracer<M> tr(S, C, K, P, perm);
   tr.noise_skip = true;  // fill_hit's own noise step: computed below instead
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, 
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce, int32_t* noise_count, int32_t* bin_count) {
-  Tracer<M> tr(S, C, K, P, kPerm);
+  Tracer<M> tr(S, C, K, P, kPermTable.v);
   const bool sky_active = K.sky_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     bool noisy = false;
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, 
 template <int M>
 __global__ __launch_bounds__(256) void wf_film(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W, int clear,
                                                int last) {
-  __shared__ int perm[512];
+  __shared__ int perm[kPermLen];
   if (P.wireframes == 1) stage_perm(perm);
   const int x = blockIdx.x * 16 + threadIdx.x;
   const int y = blockIdx.y * 16 + threadIdx.y;
