@@ -1261,6 +1261,10 @@ int df_update(df_ctx* c, int32_t w, int32_t h, int32_t local_size_hint) {
   const size_t n = static_cast<size_t>(w) * h;
   free_paths(c);
   reset_tuning(c);
+  // no frame until every buffer has the new size: a failed allocation below
+  // leaves the handle without one (DF_ESTATE) rather than with mixed sizes
+  c->width = 0;
+  c->height = 0;
   int rc;
   if ((rc = alloc_buf(c, c->film, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->film_samples, n * sizeof(int32_t))) ||
       (rc = alloc_buf(c, c->image_float, n * 3 * sizeof(float))) || (rc = alloc_buf(c, c->image_bgra, n * 4)) ||
