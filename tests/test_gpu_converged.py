@@ -1,8 +1,9 @@
 """GPU: fast-math renders against exact-math renders of the same workloads with independent random streams.
 
 Exact math is the oracle's arithmetic: its renders equal the oracle's bit for bit on the same seeds (the
-exact-mode parity tests, and the full-size rows below), so an exact-math render on the GPU stands for the
-oracle's where the oracle on the CPU would take hours.  Bars (the fast-math statistics a user of the
+exact-mode parity tests, and below the sphere workload's whole frame and sampled rows of the others at
+their full sizes), so an exact-math render on the GPU stands for the oracle's where the oracle on the CPU
+would take hours.  Bars (the fast-math statistics a user of the
 reference can expect):
 * full size (1024 x 768 and 1920 x 1080), 64 samples per pixel: per-pixel means of the linear film agree
   within 3 sigma of their sample variance on all but a few per mille of the channels;
@@ -33,7 +34,33 @@ def _params(sc, mode):
     return p
 
 
-@pytest.mark.parametrize("cid", [2, 3])
+def test_exact_math_whole_frame_of_the_sphere_workload(scenes, gpu):
+    """The 256 x 256 spheres-and-plane workload at its full size, every pixel: film, samples, seeds and BGRA
+    after 3 passes of both path-tracer schedules equal the oracle's."""
+    sc = scenes(1)
+    p = _params(sc, dfa.MATH_EXACT)
+    for engine in (dfa.PT_MEGAKERNEL, dfa.PT_WAVEFRONT):
+        p.pt_engine = engine
+        k = dfa.HIPRendererKernel(sc)
+        try:
+            k.setParams(p)
+            k.update(sc.width, sc.height)
+            k.seed(13)
+            seeds0 = k.getSeeds()
+            k.execute(3)
+            film, n = k.getFilm()
+            seeds, bgra = k.getSeeds(), k.getImageBGRA()
+        finally:
+            k.dispose()
+        st = oracle_ffi.OracleState(sc.width, sc.height, seeds0)
+        oracle_ffi.render(sc, p, st, 3)
+        np.testing.assert_array_equal(film, st.film)
+        np.testing.assert_array_equal(n, st.film_samples)
+        np.testing.assert_array_equal(seeds, st.seeds)
+        np.testing.assert_array_equal(bgra, st.image_bgra)
+
+
+@pytest.mark.parametrize("cid", [2, 3, 5])
 def test_exact_math_at_full_size_is_the_oracle_on_sampled_rows(scenes, gpu, cid):
     sc = scenes(cid)
     p = _params(sc, dfa.MATH_EXACT)
