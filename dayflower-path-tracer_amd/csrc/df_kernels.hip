@@ -134,8 +134,11 @@ __global__ __launch_bounds__(256, 6) void batch_kernel(const DJob* __restrict__ 
     const DBatchItem item = items[it];
     // jobs is read-only and restrict: its fields are scalar loads, as kernel arguments are
     const DJob& J = jobs[item.job];
-    const int x = item.tile_x * 16 + threadIdx.x;
-    const int y = item.tile_y * 16 + threadIdx.y;
+    // a wave covers an 8x8 quadrant of the tile, a more coherent set of rays than a 16x4 strip (the first
+    // 128 scenes of the batch mix 1.21 -> 1.13 s; all 256 unchanged)
+    const int t = threadIdx.y * 16 + threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int x = item.tile_x * 16 + (wave & 1) * 8 + (lane & 7);
+    const int y = item.tile_y * 16 + (wave >> 1) * 8 + (lane >> 3);
     unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (x < J.P.width && y < J.P.height) render_pixel<M, R>(J.S, J.C, J.K, J.P, J.F, perm, x, y, cnt);
     flush_counters(cnt, J.F.counters, kStageMega);
