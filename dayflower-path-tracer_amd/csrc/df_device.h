@@ -408,8 +408,8 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
   int last = -1;
   unsigned evals = 0;
   int k = 0;
+  float tc = n > 0 ? tc_tab[0] : 0.0f;  // tc_tab[k]
   while (k < n) {
-    const float tc = tc_tab[k];
     if (tc * 0.98f > closest) break;
     ++evals;
     const float px = bx + dx * tc;
@@ -435,20 +435,38 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
     }
     if (dy >= 0.0f && py > 1.0f) break;
     const float g = py - h;
-    int m = 0;
-    if (g > margin) {
-      const float thr = tc + (g - margin) * inv_l;
-      // tc_{k+m} ~ tc_k * 1.01^m: a first guess, corrected against the table
-      float mf = __log2f(thr / tc) * 69.66071689357483f;
-      mf = fminf(fmaxf(mf, 0.0f), static_cast<float>(n - 1 - k));
-      m = static_cast<int>(mf);
-      while (m > 0 && tc_tab[k + m] > thr) --m;
-      while (k + m + 1 < n && tc_tab[k + m + 1] <= thr) ++m;
-    }
     lh = h;
     ly = py;
     last = k;
-    k += m + 1;
+    // The next position to evaluate: k + 1, or after a step with g > margin
+    // the first j > k with tc_j > thr (every position up to thr is skipped).
+    int j = k + 1;
+    if (g > margin) {
+      const float thr = tc + (g - margin) * inv_l;
+      // tc_{k+m} ~ tc_k * 1.01^m: a first guess (approximate reciprocal and
+      // logarithm; it only sets where the search starts), then the table
+      // entries on both sides of it, loaded together, and a walk from there
+      // when the guess was off.
+      float mf = __log2f(thr * __builtin_amdgcn_rcpf(tc)) * 69.66071689357483f;
+      mf = fminf(fmaxf(mf, 0.0f), static_cast<float>(n - 1 - k));
+      j = k + 1 + static_cast<int>(mf);
+      float lo = j - 1 > k ? tc_tab[j - 1] : tc;
+      float hi = j < n ? tc_tab[j] : __builtin_inff();
+      while (j - 1 > k && lo > thr) {
+        --j;
+        hi = lo;
+        lo = j - 1 > k ? tc_tab[j - 1] : tc;
+      }
+      while (j < n && hi <= thr) {
+        ++j;
+        hi = j < n ? tc_tab[j] : __builtin_inff();
+      }
+      k = j;
+      tc = hi;
+      continue;
+    }
+    k = j;
+    if (k < n) tc = tc_tab[k];
   }
   *steps += evals;
   return t > DF_EPS ? t : DF_INF;

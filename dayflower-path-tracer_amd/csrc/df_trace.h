@@ -266,6 +266,7 @@ struct Tracer {
       const int m = n - base < 64 ? n - base : 64;
       uint64_t cand = 0, any = 0;  // this lane's marks; the wave's union
       for (int j0 = 0; j0 < m; j0 += 8) {  // the sphere array is padded to a multiple of 8
+        unsigned c8 = 0;                     // marks of primitives j0 .. j0 + 7
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int j = j0 + u;
@@ -273,14 +274,17 @@ struct Tracer {
           const float lx = cu.c[0] - ox, ly = cu.c[1] - oy, lz = cu.c[2] - oz;
           const float l2 = lx * lx + ly * ly + lz * lz;
           const float tca = lx * dx + ly * dy + lz * dz;
-          const float R = cu.r + 1.0e-4f * __builtin_amdgcn_sqrtf(l2) + cu.k * l2;
+          // |c - o| bounded above by its 1-norm (no square root): a larger
+          // enlargement only skips less
+          const float R = cu.r + 1.0e-4f * (fabsf(lx) + fabsf(ly) + fabsf(lz)) + cu.k * l2;
           const float d2 = l2 - tca * tca * ia;
           // comparisons are false for NaN: a ray with non-finite terms is never skipped
           const int miss = static_cast<int>(d2 > R * R) | static_cast<int>(tca < -R * dl);
           const int keep = static_cast<int>(j < m) & ((cu.on == 0) | (miss ^ 1));
-          cand |= static_cast<uint64_t>(keep) << j;
+          c8 |= static_cast<unsigned>(keep) << u;
           any |= __ballot(keep) != 0 ? uint64_t{1} << j : uint64_t{0};
         }
+        cand |= static_cast<uint64_t>(c8) << j0;
       }
       // testing: phase 0 the primitives other than meshes, phase 1 the mesh runs
       const uint64_t marked = any;
