@@ -380,6 +380,9 @@ def main():
     while k.engine() == dfa.PT_AUTO:  # the autotuner times both schedules on the first passes
         k.execute(1)
         tune_passes += 1
+    # the timed passes leave the unit counters off (rays and samples are still counted); the passes whose
+    # kernel times the roofline uses are counted again afterwards from the same seeds
+    k.setUnitCounters(False)
     for _ in range(args.warmup):
         k.execute(1)
     k.synchronize()
@@ -410,8 +413,11 @@ def main():
     te1 = time.perf_counter()
     k.setGraphs(False)
 
-    # per-stage kernel times: a separate run with HIP events around every launch
+    # per-stage kernel times: a separate run with HIP events around every launch, then the same passes again
+    # from the same seeds with the unit counters on (the work of a pass is a function of the scene, the
+    # parameters and the seeds), for the counts the roofline divides by those times
     stage_steps = eager_steps
+    seeds_stage = k.getSeeds()
     k.setStageTiming(True)
     k.resetCounters()
     for _ in range(stage_steps):
@@ -419,6 +425,16 @@ def main():
     k.synchronize()
     stages = k.stageStats()
     k.setStageTiming(False)
+    k.setSeeds(seeds_stage)
+    k.setUnitCounters(True)
+    k.resetCounters()
+    for _ in range(stage_steps):
+        k.execute(1)
+    k.synchronize()
+    counted = k.stageStats()
+    for n in stages:
+        assert counted[n]["counters"]["rays"] == stages[n]["counters"]["rays"], n  # the same passes
+        stages[n]["counters"] = counted[n]["counters"]
     k.dispose()
 
     wall = t1 - t0

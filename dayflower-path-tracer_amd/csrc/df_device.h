@@ -393,7 +393,7 @@ __device__ __forceinline__ float isect_triangle(float ox, float oy, float oz, fl
 // * h <= 1, so a ray with dy >= 0 that is above py = 1 never crosses;
 // * the interpolation needs (h, py) at the step before the crossing: they
 //   are recomputed there when that step was skipped.
-// `steps` counts height evaluations.
+// `steps` (if not null) counts height evaluations.
 template <int M>
 __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, float dx, float dy, float dz,
                                                float closest, const float* __restrict__ tc_tab, int n,
@@ -468,7 +468,7 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
     k = j;
     if (k < n) tc = tc_tab[k];
   }
-  *steps += evals;
+  if (steps) *steps += evals;
   return t > DF_EPS ? t : DF_INF;
 }
 

@@ -35,16 +35,16 @@ static_assert(sizeof(DPrim) == 184, "DPrim layout");
 // World-space bounding sphere of what a primitive's test can hit, for
 // skipping primitives a ray cannot reach.  on = 0: never skipped (planes,
 // terrains, single triangles).  A ray skips the primitive only when the
-// sphere enlarged by R = r + 1e-4 |c - o| + k |c - o|^2 lies off its line,
-// wholly behind its origin or wholly beyond the closest hit so far.  k =
-// 1e-6 s / r, s <= 16 the squared stretch ratio of the primitive's
-// transform: the enlargement exceeds the float error of the object-space
-// sphere and box tests (about 1e-7 s |c - o|^2 / r and 1e-7 |c - o|) and of
-// this test by an order of magnitude, so a skipped primitive is one whose
-// own test would report no hit closer than the closest one.
+// sphere of radius R = r + k |c - o|^2 lies off its line or wholly behind
+// its origin.  R is at least the sphere's radius enlarged by 1e-4 |c - o| +
+// 1e-6 s / r' |c - o|^2 (r' the sphere's own radius, s <= 16 the squared
+// stretch ratio of the primitive's transform), which exceeds the float
+// error of the object-space sphere and box tests (about 1e-7 s |c - o|^2 /
+// r' and 1e-7 |c - o|) and of this test by an order of magnitude, so a
+// skipped primitive is one whose own test would report no hit.
 struct DCull {
   float c[3];
-  float r;
+  float r;  // R = r + k |c - o|^2
   float k;
   int32_t on;
   int32_t pad[2];
@@ -149,6 +149,9 @@ struct DParams {
   int32_t shade_binned;
   // megakernel schedule: 0 = a wave covers a 16x4 strip of its 16x16 tile, 1 = an 8x8 quadrant
   int32_t wave_tile;
+  // wavefront schedule: 0 = its trace kernels leave the unit counters (box,
+  // triangle, primitive and sphere tests, terrain heights) at zero
+  int32_t count_units;
   // sRGB constants (image4j RGBColorSpace.SRGB)
   float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
   // differentiable mode (diff_on = 1): continuous albedo / emission

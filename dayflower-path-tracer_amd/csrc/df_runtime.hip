@@ -105,6 +105,9 @@ struct df_ctx {
   bool graphs = false;
   GraphCache g_wave[2][2], g_mega;
 
+  // the wavefront trace kernels count units (df_set_unit_counters)
+  bool count_units = true;
+
   // path-tracer schedule: fixed by df_set_schedule (sched_fixed), else for
   // DF_PT_AUTO the autotuner's choice once `tuned`, else the default
   // schedule of params.pt_engine (current_schedule)
@@ -489,7 +492,14 @@ DCull cull_bounds(const HostScene& h, const std::vector<DNode>& nodes, const DPr
   for (int k = 0; k < 3; ++k) ulp += std::fabs(cw[k] - out.c[k]);
   out.r = std::nextafter(static_cast<float>(rw + ulp), INFINITY);
   // the sphere test's float error in world units is below 1e-7 (e_max / e_min) |c - o|^2 / rw
-  out.k = std::nextafter(static_cast<float>(1e-6 * (e_max / e_min) / rw), INFINITY);
+  const double k = 1e-6 * (e_max / e_min) / rw;
+  // The kernel enlarges the radius to rw + 1e-4 |c - o| + k |c - o|^2 (ten times the float error of the
+  // box and sphere tests); with 1e-4 |c - o| <= 5e-5 (|c - o|^2 / s + s) for any s > 0 it evaluates
+  // a + b |c - o|^2 >= that, a = rw + 5e-5 s and b = k + 5e-5 / s, with s the distance of the sphere from
+  // the origin (equality at |c - o| = s).
+  const double s = std::sqrt(cw[0] * cw[0] + cw[1] * cw[1] + cw[2] * cw[2]) + rw + 1.0;
+  out.r = std::nextafter(static_cast<float>(static_cast<double>(out.r) + 5e-5 * s), INFINITY);
+  out.k = std::nextafter(static_cast<float>(k + 5e-5 / s), INFINITY);
   if (!std::isfinite(out.r) || !std::isfinite(out.k)) return DCull{};
   out.on = 1;
   return out;
@@ -645,6 +655,7 @@ DParams make_params(const df_params& p, int w, int h, int passes, bool clear) {
   d.height = h;
   d.passes = passes;
   d.clear_first = clear ? 1 : 0;
+  d.count_units = 1;
   // image4j RGBColorSpace.SRGB (breakPoint 0.00304, gamma 2.4) and its
   // derived slope / slopeMatch / segmentOffset (RGBColorSpace.java:148-158).
   const float bp = 0.00304f, gamma = 2.4f;
@@ -1109,6 +1120,7 @@ int batch_launch(df_batch* b, const BatchPlan& plan, const DJob* djobs, const DB
     df_ctx* c = b->jobs[j];
     DParams par = plan.jobs[j].P;
     par.shade_binned = current_schedule(c).shade_binned;
+    par.count_units = c->count_units ? 1 : 0;
     DF_BHIP(b, dfk_launch_pt_wavefront(c->ds, plan.jobs[j].C, plan.jobs[j].K, par, plan.jobs[j].F, c->paths,
                                        c->params.math_mode, b->stream, nullptr));
   }
@@ -1439,6 +1451,7 @@ int df_execute(df_ctx* c, int32_t passes) {
     apply_diff(c, par);
     par.shade_binned = wave && sc.shade_binned ? 1 : 0;
     par.wave_tile = sc.wave_tile;
+    par.count_units = c->count_units ? 1 : 0;
     const int math = c->params.math_mode;
     if (wave && graph) {
       // one graph per pass; the wavefront kernels read neither passes nor clear_first
@@ -1651,6 +1664,12 @@ int df_set_graphs(df_ctx* c, int32_t enabled) {
       for (GraphCache& g : row) g.reset();
     c->g_mega.reset();
   }
+  return DF_OK;
+}
+
+int df_set_unit_counters(df_ctx* c, int32_t enabled) {
+  if (!c) return DF_EINVAL;
+  c->count_units = enabled != 0;
   return DF_OK;
 }
 

@@ -36,7 +36,11 @@ struct Vtx {
 // 1/6, 1/2, 3/4 and all slower).
 constexpr int kLeafWaitNum = 1, kLeafWaitDen = 4;
 
-template <int M>
+// kUnits: count the unit counters (box, triangle, primitive and sphere
+// tests, terrain height evaluations); the wavefront's trace kernels also come
+// without them, for callers that need only rays and samples (about 3 % of
+// those kernels' time).
+template <int M, bool kUnits = true>
 struct Tracer {
   const DScene& S;
   const DCamera& C;
@@ -62,6 +66,10 @@ struct Tracer {
   unsigned sphere_tests;
   unsigned terrain_steps;
   unsigned sky_evals;
+
+  __device__ __forceinline__ static void unit(unsigned& counter) {
+    if constexpr (kUnits) ++counter;
+  }
 
   __device__ Tracer(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const int* perm)
       : S(s), C(c), K(k), P(p), rays(0), samples(0), box_tests(0), tri_tests(0), prim_tests(0),
@@ -171,7 +179,7 @@ struct Tracer {
         if (leaf >= 0) {
           for (int k = leaf;; ++k) {
             const DTri tr = at(S.tris, k);
-            ++tri_tests;
+            unit(tri_tests);
             const float cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
             if (cur < closest) {
               closest = cur;
@@ -191,7 +199,7 @@ struct Tracer {
         if (!mine) break;
         pi = base + __builtin_ctzll(mine);
         mine &= mine - 1;
-        ++prim_tests;
+        unit(prim_tests);
         const DPrim& pr = S.prims[pi];
         idx = pr.node_root;
         // The object-space ray and its reciprocals, unless the previous mesh
@@ -213,7 +221,7 @@ struct Tracer {
         }
       }
       const DNode nd = at(S.nodes, idx);
-      ++box_tests;
+      unit(box_tests);
       const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
       const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
       const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
@@ -274,9 +282,9 @@ struct Tracer {
           const float lx = cu.c[0] - ox, ly = cu.c[1] - oy, lz = cu.c[2] - oz;
           const float l2 = lx * lx + ly * ly + lz * lz;
           const float tca = lx * dx + ly * dy + lz * dz;
-          // |c - o| bounded above by its 1-norm (no square root): a larger
-          // enlargement only skips less
-          const float R = cu.r + 1.0e-4f * (fabsf(lx) + fabsf(ly) + fabsf(lz)) + cu.k * l2;
+          // the enlarged radius r + 1e-4 |c - o| + k |c - o|^2, bounded above
+          // without a square root as a + b |c - o|^2 (DCull)
+          const float R = cu.r + cu.k * l2;
           const float d2 = l2 - tca * tca * ia;
           // comparisons are false for NaN: a ray with non-finite terms is never skipped
           const int miss = static_cast<int>(d2 > R * R) | static_cast<int>(tca < -R * dl);
@@ -319,7 +327,7 @@ struct Tracer {
           if (!((cand >> j) & 1)) continue;
           const int i = base + j;
           const int st = pr.type;
-          ++prim_tests;
+          unit(prim_tests);
           const float* m = pr.w2o;
           const float oxo = m[0] * ox + m[1] * oy + m[2] * oz + m[3];
           const float oyo = m[4] * ox + m[5] * oy + m[6] * oz + m[7];
@@ -332,13 +340,14 @@ struct Tracer {
             cur = isect_plane(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], pr.geo[4],
                               pr.geo[5], closest);
           } else if (st == 2) {
-            ++sphere_tests;
+            unit(sphere_tests);
             cur = isect_sphere(oxo, oyo, ozo, dxo, dyo, dzo, pr.geo[0], pr.geo[1], pr.geo[2], pr.geo[3], closest);
           } else if (st == 3) {
-            cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n, &terrain_steps);
+            cur = isect_terrain<M>(oxo, oyo, ozo, dxo, dyo, dzo, closest, S.march_tc, S.march_n,
+                                   kUnits ? &terrain_steps : nullptr);
           } else if (st == 4) {
             const DTri tr = S.tris[pr.node_root];
-            ++tri_tests;
+            unit(tri_tests);
             cur = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
           }
           if (cur < closest) {

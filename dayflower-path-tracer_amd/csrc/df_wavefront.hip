@@ -269,12 +269,13 @@ __global__ __launch_bounds__(kThreads) void wf_normals(DScene S, DCamera C, DSky
 // Occupancy hint: seven waves per SIMD for the bounces' queries, five for the
 // last launch's (a few registers spilled); measured faster than five, six or
 // eight for the former and six or seven for the latter.
-template <int M, bool kEscape>
+// kUnits: with the unit counters (Tracer).
+template <int M, bool kEscape, bool kUnits>
 __global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce, int32_t* noise_count, int32_t* bin_count) {
-  Tracer<M> tr(S, C, K, P, kPermTable.v);
+  Tracer<M, kUnits> tr(S, C, K, P, kPermTable.v);
   const bool sky_active = K.sky_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     bool noisy = false;
@@ -398,13 +399,15 @@ int resident_blocks(Kernel k) {
 // any graph capture: the occupancy queries are not stream work).
 struct Grids {
   int trace, shade, shade_binned, escape, normals;
+  int trace_nu, escape_nu;  // the trace kernels without unit counters
 };
 
 template <int M>
 const Grids& grids() {
-  static const Grids g{resident_blocks(wf_trace<M, false>), resident_blocks(wf_shade<M, false>),
-                       resident_blocks(wf_shade<M, true>), resident_blocks(wf_trace<M, true>),
-                       resident_blocks(wf_normals<M>)};
+  static const Grids g{resident_blocks(wf_trace<M, false, true>), resident_blocks(wf_shade<M, false>),
+                       resident_blocks(wf_shade<M, true>),        resident_blocks(wf_trace<M, true, true>),
+                       resident_blocks(wf_normals<M>),             resident_blocks(wf_trace<M, false, false>),
+                       resident_blocks(wf_trace<M, true, false>)};
   return g;
 }
 
@@ -415,8 +418,11 @@ template <int M>
 hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& k, const DParams& p, const DFrame& f,
                                  const DPaths& w, hipStream_t stream, const dfk_marks* marks, int clear, int last) {
   const Grids& g = grids<M>();
-  const int g_trace = g.trace, g_shade = g.shade, g_shade_binned = g.shade_binned, g_escape = g.escape,
-            g_normals = g.normals;
+  const bool units = p.count_units != 0;
+  const int g_trace = units ? g.trace : g.trace_nu, g_shade = g.shade, g_shade_binned = g.shade_binned,
+            g_escape = units ? g.escape : g.escape_nu, g_normals = g.normals;
+  const auto trace_kernel = units ? wf_trace<M, false, true> : wf_trace<M, false, false>;
+  const auto escape_kernel = units ? wf_trace<M, true, true> : wf_trace<M, true, false>;
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
@@ -447,7 +453,7 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
       // shadow queries of bounce b - 1 (none before the first bounce: that count stays 0)
       int32_t* sc = b > 0 ? shadow_count(b - 1) : shadow_count(bounces);
       mark(kStageTrace, 1);
-      hipLaunchKernelGGL((wf_trace<M, false>), dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
+      hipLaunchKernelGGL(trace_kernel, dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
                          head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b),
                          bin_count(b));
       mark(kStageTrace, 0);
@@ -468,7 +474,7 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
     // took every bounce, in one launch (they write different path data; the
     // film pass adds the two)
     mark(kStageEscape, 1);
-    hipLaunchKernelGGL((wf_trace<M, true>), dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
+    hipLaunchKernelGGL(escape_kernel, dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
                        shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], live(bounces),
                        head(3 * bounces + 1), 0, nullptr, nullptr);
     mark(kStageEscape, 0);

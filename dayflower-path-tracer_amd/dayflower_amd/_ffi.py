@@ -170,6 +170,7 @@ HIP_PROTOTYPES = {
     "df_get_counters": (ctypes.c_int, [_P, ctypes.POINTER(Counters)]),
     "df_reset_counters": (ctypes.c_int, [_P]),
     "df_set_stage_timing": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "df_set_unit_counters": (ctypes.c_int, [_P, ctypes.c_int32]),
     "df_get_stage_stats": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(Counters),
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]),
     "df_last_error": (ctypes.c_char_p, [_P]),

@@ -338,6 +338,12 @@ class HIPRendererKernel:
                     "df_loss_gradient")
         return loss.value, grad, fms.value, (ams.value if gradient else None)
 
+    def setUnitCounters(self, enabled):
+        """df_set_unit_counters: with False the wavefront's trace kernels skip the unit counters (box, triangle,
+        primitive and sphere tests, terrain heights), about 3 % of their time; rays and samples are still counted
+        and the results are the same bits."""
+        self._check(self._lib.df_set_unit_counters(self._ctx, 1 if enabled else 0), "df_set_unit_counters")
+
     def setStageTiming(self, enabled):
         """Brackets every kernel launch with HIP events (see stageStats)."""
         self._check(self._lib.df_set_stage_timing(self._ctx, 1 if enabled else 0), "df_set_stage_timing")

@@ -151,6 +151,15 @@ int df_set_film(df_ctx* ctx, const float* rgb, size_t rgb_n, const int32_t* samp
 int df_get_counters(df_ctx* ctx, df_counters* out);
 int df_reset_counters(df_ctx* ctx);
 
+/* Unit counters (no counterpart in the reference): with them off (default
+ * on), the wavefront schedule's trace kernels leave box_tests, tri_tests,
+ * prim_tests, sphere_tests and terrain_steps at zero (rays, samples and
+ * sky_evals are still counted), which saves about 3 % of their time.  The
+ * counts are a function of the scene, parameters and seeds, so a caller can
+ * time passes without them and count the same passes again (df_set_seeds)
+ * with them.  Results are the same bits either way. */
+int df_set_unit_counters(df_ctx* ctx, int32_t enabled);
+
 /* Per-stage instrumentation (no counterpart in the reference, whose kernel
  * is one Aparapi launch).  With timing on, every kernel launch is bracketed
  * by HIP events on the handle's stream; df_get_stage_stats returns, for one
