@@ -387,6 +387,13 @@ __device__ __forceinline__ float isect_triangle(float ox, float oy, float oz, fl
 //   rounding of py and h at both steps (positions carry ~2^-23 relative
 //   error, sin ~1 ulp; it keeps the coarser |dx|+|dy|+|dz| for the
 //   positions' error), with a wide safety factor;
+// * it also skips to the positive root of g + g' s - D2 s^2 / 2 (a lower
+//   bound of the gap s past tc, from |g''| <= D2 = dx^2 + dz^2 and a lower
+//   bound on g' at tc) when that is farther: rays leaving the surface or
+//   skimming it, whose gap grows or stays small, otherwise advance by tiny
+//   first-order steps (1080p pass of the mesh + terrain workload: 193 M ->
+//   57 M height evaluations).  df_check_terrain_march compares this march
+//   with the plain one on millions of rays;
 // * a crossing at step j returns t >= tc_{j-1} (rounded), so once
 //   tc_{k-1} > closest the terrain cannot win the closest-hit comparison
 //   (tc_{k-1} >= 0.98 tc_k);
@@ -404,6 +411,9 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
   const float L = (fabsf(dy) + sqrtf(dx * dx + dz * dz)) * 1.00000095367431640625f;
   const float margin = 1.0e-5f + 9.5367431640625e-07f * (fabsf(bx) + fabsf(by) + fabsf(bz) + 10.0f * L1);
   const float inv_l = 1.0f / L;
+  // second-order bound: |g''| <= D2 = dx^2 + dz^2 (rounded up, halved)
+  const float qb = (dx * dx + dz * dz) * 0.500005f;
+  const float gp_err = 1.0e-5f * L1;
   float lh = 0.0f, ly = 0.0f, t = 0.0f;
   int last = -1;
   unsigned evals = 0;
@@ -415,7 +425,8 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
     const float px = bx + dx * tc;
     const float py = by + dy * tc;
     const float pz = bz + dz * tc;
-    const float h = Math<M>::sin(px) * Math<M>::sin(pz);
+    const float sx = Math<M>::sin(px), sz = Math<M>::sin(pz);
+    const float h = sx * sz;
     if (py < h) {
       float td = 0.01f;
       if (k > 0) {
@@ -442,7 +453,27 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
     // the first j > k with tc_j > thr (every position up to thr is skipped).
     int j = k + 1;
     if (g > margin) {
-      const float thr = tc + (g - margin) * inv_l;
+      const float gm = g - margin;
+      float skip = gm * inv_l;
+      if (qb > 0.0f) {
+        // g(tc + s) >= g + g' s - D2 s^2 / 2, with g' = dy - (dx cos px sin pz
+        // + dz sin px cos pz) >= dy - |dx| |cos px| |sin pz| - |dz| |sin px|
+        // |cos pz| - gp_err (|cos| bounded above through 1 - sin^2 with room
+        // for the sines' rounding, gp_err the rounding of this sum), and the
+        // directional second derivative of sin x sin z is at most D2 in
+        // magnitude (|sin x sin z| + |cos x cos z| <= 1).  Its positive root,
+        // cancellation-free and shrunk past its own rounding, is a second
+        // skip distance; rays climbing away from the surface or skimming it
+        // take far longer skips than the first-order one.
+        const float cx = __builtin_amdgcn_sqrtf(1.0f - sx * sx + 2.0e-6f);
+        const float cz = __builtin_amdgcn_sqrtf(1.0f - sz * sz + 2.0e-6f);
+        const float gp = dy - fabsf(dx) * cx * fabsf(sz) - fabsf(dz) * fabsf(sx) * cz - gp_err;
+        const float sq = __builtin_amdgcn_sqrtf(gp * gp + 4.0f * qb * gm);
+        const float s2 = gp > 0.0f ? (gp + sq) * __builtin_amdgcn_rcpf(2.0f * qb)
+                                   : 2.0f * gm * __builtin_amdgcn_rcpf(sq - gp);
+        skip = fmaxf(skip, s2 * 0.99999f);
+      }
+      const float thr = tc + skip;
       // tc_{k+m} ~ tc_k * 1.01^m: a first guess (approximate reciprocal and
       // logarithm; it only sets where the search starts), then the table
       // entries on both sides of it, loaded together, and a walk from there

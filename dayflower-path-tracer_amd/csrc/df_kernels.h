@@ -23,6 +23,11 @@ hipError_t dfk_launch_batch(const dfk::DJob* jobs, const dfk::DBatchItem* items,
 
 hipError_t dfk_launch_seed(uint64_t* seeds, uint64_t n, uint64_t master, hipStream_t stream);
 
+// n rays (6 floats each) through the terrain march of the kernels and
+// through the reference's march over every position (df_check_terrain_march).
+hipError_t dfk_launch_terrain_check(const float* rays, int64_t n, const float* tc_tab, int tn, float* t_skip,
+                                    float* t_plain, uint32_t* evals, int math_mode, hipStream_t stream);
+
 // One or more passes of the path tracer as wavefront kernels (df_wavefront.hip);
 // w must hold at least width * height slots and wavefront_ctl_len(depth) counters.
 hipError_t dfk_launch_pt_wavefront(const dfk::DScene& s, const dfk::DCamera& c, const dfk::DSky& k,

@@ -202,6 +202,57 @@ hipError_t dfk_launch_batch(const dfk::DJob* jobs, const dfk::DBatchItem* items,
   return hipGetLastError();
 }
 
+namespace dfk {
+
+// The reference's terrain march (GPURendererKernel.java:1470-1506) step by
+// step over every position of the table, for checking isect_terrain.
+template <int M>
+__device__ float march_every_position(float ox, float oy, float oz, float dx, float dy, float dz, const float* tc_tab,
+                                      int n) {
+  const float sr = 1.0f / 10.0f;
+  float t = 0.0f, td = 0.01f, lh = 0.0f, ly = 0.0f;
+  for (int k = 0; k < n; ++k) {
+    const float tc = tc_tab[k];
+    const float px = ox * sr + dx * tc, py = oy * sr + dy * tc, pz = oz * sr + dz * tc;
+    const float h = Math<M>::sin(px) * Math<M>::sin(pz);
+    if (py < h) {
+      t = tc - td + td * (lh - ly) / (py - ly - h + lh);
+      break;
+    }
+    td = 0.01f * tc;
+    lh = h;
+    ly = py;
+  }
+  return t > DF_EPS ? t : DF_INF;
+}
+
+template <int M>
+__global__ void terrain_check_kernel(const float* rays, int64_t n, const float* tc_tab, int tn, float* t_skip,
+                                     float* t_plain, uint32_t* evals) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* r = rays + 6 * i;
+  unsigned e = 0;
+  t_skip[i] = isect_terrain<M>(r[0], r[1], r[2], r[3], r[4], r[5], DF_INF, tc_tab, tn, &e);
+  evals[i] = e;
+  t_plain[i] = march_every_position<M>(r[0], r[1], r[2], r[3], r[4], r[5], tc_tab, tn);
+}
+
+}  // namespace dfk
+
+hipError_t dfk_launch_terrain_check(const float* rays, int64_t n, const float* tc_tab, int tn, float* t_skip,
+                                    float* t_plain, uint32_t* evals, int math_mode, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  const unsigned blocks = static_cast<unsigned>((n + 255) / 256);
+  if (math_mode == 1)
+    hipLaunchKernelGGL(dfk::terrain_check_kernel<1>, dim3(blocks), dim3(256), 0, stream, rays, n, tc_tab, tn, t_skip,
+                       t_plain, evals);
+  else
+    hipLaunchKernelGGL(dfk::terrain_check_kernel<0>, dim3(blocks), dim3(256), 0, stream, rays, n, tc_tab, tn, t_skip,
+                       t_plain, evals);
+  return hipGetLastError();
+}
+
 hipError_t dfk_launch_seed(uint64_t* seeds, uint64_t n, uint64_t master, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   const unsigned blocks = static_cast<unsigned>((n + 255) / 256);

@@ -1667,6 +1667,39 @@ int df_set_graphs(df_ctx* c, int32_t enabled) {
   return DF_OK;
 }
 
+int df_check_terrain_march(int32_t device, int32_t math_mode, const float* rays, size_t n, float* t_skip,
+                           float* t_plain, uint32_t* evals) {
+  if (!rays || !t_skip || !t_plain || !evals) return DF_EINVAL;
+  if (math_mode != DF_MATH_FAST && math_mode != DF_MATH_EXACT) return DF_EINVAL;
+  if (n == 0) return DF_OK;
+  if (n > (size_t{1} << 26)) return DF_EINVAL;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return DF_ENODEVICE;
+  if (device < 0 || device >= count) return DF_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return DF_EHIP;
+  const std::vector<float> tab = terrain_march_positions();
+  float *d_rays = nullptr, *d_tab = nullptr, *d_skip = nullptr, *d_plain = nullptr;
+  uint32_t* d_evals = nullptr;
+  int rc = DF_OK;
+  if (hipMalloc(&d_rays, n * 6 * sizeof(float)) != hipSuccess || hipMalloc(&d_tab, tab.size() * sizeof(float)) != hipSuccess ||
+      hipMalloc(&d_skip, n * sizeof(float)) != hipSuccess || hipMalloc(&d_plain, n * sizeof(float)) != hipSuccess ||
+      hipMalloc(&d_evals, n * sizeof(uint32_t)) != hipSuccess) {
+    rc = DF_ENOMEM;
+  } else if (hipMemcpy(d_rays, rays, n * 6 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
+             dfk_launch_terrain_check(d_rays, static_cast<int64_t>(n), d_tab, static_cast<int>(tab.size()), d_skip,
+                                      d_plain, d_evals, math_mode, nullptr) != hipSuccess ||
+             hipMemcpy(t_skip, d_skip, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(t_plain, d_plain, n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(evals, d_evals, n * sizeof(uint32_t), hipMemcpyDeviceToHost) != hipSuccess) {
+    rc = DF_EHIP;
+  }
+  for (void* p : {static_cast<void*>(d_rays), static_cast<void*>(d_tab), static_cast<void*>(d_skip),
+                  static_cast<void*>(d_plain), static_cast<void*>(d_evals)})
+    if (p) (void)hipFree(p);
+  return rc;
+}
+
 int df_set_unit_counters(df_ctx* c, int32_t enabled) {
   if (!c) return DF_EINVAL;
   c->count_units = enabled != 0;

@@ -69,6 +69,23 @@ def measure_roof(kind, device=0):
     return rate.value, work.value
 
 
+def check_terrain_march(rays, math_mode=None, device=0):
+    """df_check_terrain_march: (t_skip, t_plain, evals) of the kernels' terrain march and of the reference's march
+    over every position for rays of shape (n, 6) (object-space origin, direction)."""
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    t_skip = np.empty(n, np.float32)
+    t_plain = np.empty(n, np.float32)
+    evals = np.empty(n, np.uint32)
+    mode = MATH_FAST if math_mode is None else math_mode
+    rc = _ffi.hip_lib().df_check_terrain_march(
+        int(device), int(mode), rays.ctypes.data_as(_ffi.c_float_p), n, t_skip.ctypes.data_as(_ffi.c_float_p),
+        t_plain.ctypes.data_as(_ffi.c_float_p), evals.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    if rc != _ffi.DF_OK:
+        raise HIPError(rc, "df_check_terrain_march failed")
+    return t_skip, t_plain, evals
+
+
 class HIPRendererKernel:
     """Renderer bound to one compiled scene on one HIP device."""
 

@@ -244,6 +244,18 @@ int df_batch_set_graphs(df_batch* batch, int32_t enabled);
 int df_batch_last_execute_ms(df_batch* batch, float* ms);
 const char* df_batch_last_error(df_batch* batch);
 
+/* ---- Terrain march check --------------------------------------------------
+ * (no counterpart in the reference: a self-check for tests.)  Marches n rays
+ * (6 floats each: object-space origin, then direction) against the terrain
+ * height field on `device` twice: with the kernels' march, which skips the
+ * positions a crossing provably cannot lie at (t_skip, and the heights it
+ * evaluated in evals), and with the reference's march over every position
+ * (GPURendererKernel.java:1470-1506, t_plain), both in math_mode and with no
+ * closer hit.  The two distances must be equal bit for bit.  n <= 2^26;
+ * allocates and frees its own device buffers. */
+int df_check_terrain_march(int32_t device, int32_t math_mode, const float* rays, size_t n, float* t_skip,
+                           float* t_plain, uint32_t* evals);
+
 /* ---- Measured roofs -------------------------------------------------------
  * (no counterpart in the reference: measurement only.)  Runs one
  * microbenchmark on `device` and returns its sustained rate: FLOP/s of
