@@ -73,6 +73,22 @@ struct DTri {
   int32_t pad;
 };
 
+// Two-child BVH records (64 bytes, one cache line), for the walk that tests
+// both children of a tree node from one fetch and keeps the second child's
+// test result on a short per-lane stack: the boxes of both children of tree
+// node i of the DNode array, component-major (min x of child 0 and child 1,
+// min y, ...), and the children: >= 0 a tree node (its record), -1 a leaf
+// without triangles, <= -2 a leaf whose first packed triangle is -ref - 2.
+// Indexed like DNode (records of leaf nodes unused).
+constexpr int kPairStackMax = 16;  // deepest per-lane stack the kernels provide
+struct alignas(64) DPair {
+  float mn[3][2];
+  float mx[3][2];
+  int32_t ref[2];
+  int32_t pad[2];
+};
+static_assert(sizeof(DPair) == 64, "DPair layout");
+
 struct DScene {
   // unchanged reference arrays
   const float* point2fs;
@@ -93,6 +109,10 @@ struct DScene {
   // Terrain march positions tc_0 = 0.001, tc_{k+1} = tc_k + 0.01 * tc_k (float
   // arithmetic, every tc_k < 10): the sequence every terrain query visits.
   const float* march_tc;
+  // two-child records (null when some mesh's tree is not binary or needs a
+  // deeper stack than kPairStackMax) and the stack depth the scene needs
+  const DPair* pairs;
+  int32_t pair_stack;
   int32_t prim_count;
   int32_t march_n;
 };
@@ -152,6 +172,9 @@ struct DParams {
   // wavefront schedule: 0 = its trace kernels leave the unit counters (box,
   // triangle, primitive and sphere tests, terrain heights) at zero
   int32_t count_units;
+  // wavefront schedule: 1 = the bounce traces walk the meshes over the
+  // two-child records (DPair) with a per-lane stack
+  int32_t bvh_pairs;
   // sRGB constants (image4j RGBColorSpace.SRGB)
   float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
   // differentiable mode (diff_on = 1): continuous albedo / emission

@@ -118,8 +118,8 @@ __device__ __forceinline__ void flush_counters(const unsigned long long cnt[kCou
   }
 }
 
-template <int M, bool U>
-__device__ __forceinline__ void take_counters(const Tracer<M, U>& tr, unsigned long long cnt[kCounterCount]) {
+template <int M, bool U, bool P>
+__device__ __forceinline__ void take_counters(const Tracer<M, U, P>& tr, unsigned long long cnt[kCounterCount]) {
   cnt[0] += tr.rays;
   cnt[1] += tr.samples;
   cnt[2] += tr.box_tests;

@@ -28,6 +28,8 @@ using dfk::DCull;
 using dfk::DFrame;
 using dfk::DJob;
 using dfk::DNode;
+using dfk::DPair;
+using dfk::kPairStackMax;
 using dfk::DParams;
 using dfk::DPrim;
 using dfk::DScene;
@@ -718,6 +720,51 @@ std::vector<float> terrain_march_positions() {
   return v;
 }
 
+// The two-child records of every mesh's tree (DPair, indexed like the
+// nodes) and the deepest per-lane stack their walk needs: one entry per
+// first child on the path from the root to a tree node, plus one.  Returns
+// false (no records) when some tree node has other than two children or the
+// stack would be deeper than kPairStackMax.
+bool build_pairs(const std::vector<DNode>& nodes, const std::vector<DPrim>& prims, std::vector<DPair>& pairs,
+                 int32_t* stack_depth) {
+  pairs.assign(nodes.size(), DPair{});
+  *stack_depth = 0;
+  const int32_t n = static_cast<int32_t>(nodes.size());
+  auto valid = [&](int32_t i) { return i >= 0 && i < n; };
+  auto is_tree = [&](int32_t i) { return nodes[i].tri < 0 && -nodes[i].tri - 2 != nodes[i].next; };
+  auto ref_of = [&](int32_t i) -> int32_t {
+    if (nodes[i].tri >= 0) return -nodes[i].tri - 2;  // leaf
+    return is_tree(i) ? i : -1;                       // tree node, or a leaf without triangles
+  };
+  for (const DPrim& p : prims) {
+    if (p.type != 5 || !valid(p.node_root)) continue;
+    std::vector<std::pair<int32_t, int32_t>> todo;  // (tree node, first children on its path)
+    if (is_tree(p.node_root)) todo.push_back({p.node_root, 0});
+    while (!todo.empty()) {
+      const auto [i, firsts] = todo.back();
+      todo.pop_back();
+      const int32_t c0 = -nodes[i].tri - 2;
+      if (!valid(c0)) return false;
+      const int32_t c1 = nodes[c0].next;
+      if (!valid(c1) || c1 == nodes[i].next || nodes[c1].next != nodes[i].next) return false;
+      if (firsts + 1 > kPairStackMax) return false;
+      *stack_depth = std::max(*stack_depth, firsts + 1);
+      DPair& r = pairs[i];
+      const int32_t ch[2] = {c0, c1};
+      for (int k = 0; k < 2; ++k) {
+        for (int a = 0; a < 3; ++a) {
+          r.mn[a][k] = nodes[ch[k]].mn[a];
+          r.mx[a][k] = nodes[ch[k]].mx[a];
+        }
+        r.ref[k] = ref_of(ch[k]);
+      }
+      if (is_tree(c1)) todo.push_back({c1, firsts});
+      if (is_tree(c0)) todo.push_back({c0, firsts + 1});
+    }
+  }
+  return true;
+}
+
 int rebuild_device_scene(df_ctx* c) {
   std::vector<DPrim> prims;
   std::vector<DNode> nodes;
@@ -727,8 +774,14 @@ int rebuild_device_scene(df_ctx* c) {
   if (rc) return rc;
   // buffers: 0 point2fs 1 point3fs 2 vector3fs 3 planes 4 spheres 5 terrains
   // 6 triangles 7 textures 8 surfaces 9 primitives 10 prims 11 nodes 12 tris
-  // 13 terrain march positions 14 cull spheres
-  c->scene_bufs.resize(15);
+  // 13 terrain march positions 14 cull spheres 15 two-child records
+  c->scene_bufs.resize(16);
+  std::vector<DPair> pairs;
+  int32_t pair_stack = 0;
+  if (!build_pairs(nodes, prims, pairs, &pair_stack)) {
+    pairs.clear();
+    pair_stack = 0;
+  }
   const std::vector<float> march = terrain_march_positions();
   const HostScene& h = c->hs;
   if ((rc = upload(c, c->scene_bufs[0], h.point2fs)) || (rc = upload(c, c->scene_bufs[1], h.point3fs)) ||
@@ -738,7 +791,7 @@ int rebuild_device_scene(df_ctx* c) {
       (rc = upload(c, c->scene_bufs[8], h.surfaces)) || (rc = upload(c, c->scene_bufs[9], h.primitives)) ||
       (rc = upload(c, c->scene_bufs[10], prims)) || (rc = upload(c, c->scene_bufs[11], nodes)) ||
       (rc = upload(c, c->scene_bufs[12], tris)) || (rc = upload(c, c->scene_bufs[13], march)) ||
-      (rc = upload(c, c->scene_bufs[14], cull)))
+      (rc = upload(c, c->scene_bufs[14], cull)) || (rc = upload(c, c->scene_bufs[15], pairs)))
     return rc;
   DScene& d = c->ds;
   d.point2fs = static_cast<const float*>(c->scene_bufs[0].p);
@@ -756,6 +809,8 @@ int rebuild_device_scene(df_ctx* c) {
   d.tris = static_cast<const DTri*>(c->scene_bufs[12].p);
   d.march_tc = static_cast<const float*>(c->scene_bufs[13].p);
   d.cull = static_cast<const DCull*>(c->scene_bufs[14].p);
+  d.pairs = pairs.empty() ? nullptr : static_cast<const DPair*>(c->scene_bufs[15].p);
+  d.pair_stack = pair_stack;
   d.prim_count = static_cast<int32_t>(prims.size());
   d.march_n = static_cast<int32_t>(march.size());
   return DF_OK;
@@ -1120,6 +1175,7 @@ int batch_launch(df_batch* b, const BatchPlan& plan, const DJob* djobs, const DB
     df_ctx* c = b->jobs[j];
     DParams par = plan.jobs[j].P;
     par.shade_binned = current_schedule(c).shade_binned;
+    par.bvh_pairs = current_schedule(c).bvh_pairs;
     par.count_units = c->count_units ? 1 : 0;
     DF_BHIP(b, dfk_launch_pt_wavefront(c->ds, plan.jobs[j].C, plan.jobs[j].K, par, plan.jobs[j].F, c->paths,
                                        c->params.math_mode, b->stream, nullptr));
@@ -1452,6 +1508,7 @@ int df_execute(df_ctx* c, int32_t passes) {
     par.shade_binned = wave && sc.shade_binned ? 1 : 0;
     par.wave_tile = sc.wave_tile;
     par.count_units = c->count_units ? 1 : 0;
+    par.bvh_pairs = wave && sc.bvh_pairs ? 1 : 0;
     const int math = c->params.math_mode;
     if (wave && graph) {
       // one graph per pass; the wavefront kernels read neither passes nor clear_first
@@ -1541,6 +1598,13 @@ int df_get_schedule(df_ctx* c, df_schedule* out) {
   return DF_OK;
 }
 
+int df_get_bvh_pair_stack(df_ctx* c, int32_t* depth) {
+  if (!c || !depth) return DF_EINVAL;
+  if (!c->has_scene) return fail(c, DF_ESTATE, "no scene loaded");
+  *depth = c->ds.pairs ? c->ds.pair_stack : 0;
+  return DF_OK;
+}
+
 int df_set_schedule(df_ctx* c, const df_schedule* s) {
   if (!c) return DF_EINVAL;
   if (!s) {
@@ -1549,11 +1613,11 @@ int df_set_schedule(df_ctx* c, const df_schedule* s) {
     return DF_OK;
   }
   if ((s->engine != DF_PT_MEGAKERNEL && s->engine != DF_PT_WAVEFRONT) ||
-      (s->wave_tile != DF_TILE_ROWS && s->wave_tile != DF_TILE_SQUARE) || (s->shade_binned != 0 && s->shade_binned != 1))
-    return fail(c, DF_EINVAL, "schedule (engine %d, wave_tile %d, shade_binned %d)", s->engine, s->wave_tile,
-                s->shade_binned);
+      (s->wave_tile != DF_TILE_ROWS && s->wave_tile != DF_TILE_SQUARE) ||
+      (s->shade_binned != 0 && s->shade_binned != 1) || (s->bvh_pairs != 0 && s->bvh_pairs != 1))
+    return fail(c, DF_EINVAL, "schedule (engine %d, wave_tile %d, shade_binned %d, bvh_pairs %d)", s->engine,
+                s->wave_tile, s->shade_binned, s->bvh_pairs);
   c->sched = *s;
-  c->sched.reserved = 0;
   c->sched_fixed = true;
   return DF_OK;
 }

@@ -40,7 +40,9 @@ constexpr int kLeafWaitNum = 1, kLeafWaitDen = 4;
 // tests, terrain height evaluations); the wavefront's trace kernels also come
 // without them, for callers that need only rays and samples (about 3 % of
 // those kernels' time).
-template <int M, bool kUnits = true>
+// kPairs: the mesh walks go over the two-child records (walk_meshes_pairs),
+// with the per-lane stack in LDS at pstack_key / pstack_ref.
+template <int M, bool kUnits = true, bool kPairs = false>
 struct Tracer {
   const DScene& S;
   const DCamera& C;
@@ -55,6 +57,12 @@ struct Tracer {
   const float4* noise_in = nullptr;
   int noise_index = 0;
   bool noise_skip = false;
+  // the lane's stack of walk_meshes_pairs (entry k at [k * kPairStride]; LDS)
+  typedef __attribute__((address_space(3))) float lds_f;
+  typedef __attribute__((address_space(3))) int lds_i;
+  static constexpr int kPairStride = 256;
+  lds_f* pstack_key = nullptr;
+  lds_i* pstack_ref = nullptr;
   Rng rng;
   Hit h;
   float tmp[3];
@@ -239,6 +247,132 @@ struct Tracer {
     return false;
   }
 
+  // walk_meshes over the two-child records (DPair): one 64-byte fetch tests
+  // both children of a tree node.  Each child's slab test does not depend on
+  // the closest hit, only the decision does: the box is entered when its
+  // entry or exit lies in (0.001, closest) and tmin <= tmax (:1122-1134),
+  // which is `key < closest` for key = tmin if tmin > 0.001, else tmax if
+  // tmax > 0.001, else never (and never if tmin > tmax; NaN compares false
+  // throughout).  The first child is decided at once; the second child's key
+  // waits on the lane's stack (when it can still be entered: closest only
+  // decreases) until the first child's subtree is done and is then compared
+  // with the closest hit of that moment, so every box is decided where the
+  // pre-order skip walk tests it, against the same closest hit, and the
+  // triangles are tested in the same order.  The mesh's root is tested from
+  // its DNode.  The host provides these records only when every tree node has
+  // two children and the stack never holds more than S.pair_stack entries.
+  __device__ __forceinline__ static float pair_key(float tmin, float tmax) {
+    const float never = __builtin_inff();
+    const float k = tmin > 0.001f ? tmin : (tmax > 0.001f ? tmax : never);
+    return tmin > tmax ? never : k;
+  }
+
+  __device__ __forceinline__ bool walk_meshes_pairs(uint64_t mine, int base, float ox, float oy, float oz,
+                                                    float dx, float dy, float dz, float& closest, int& ci, int& cst,
+                                                    int& cso, bool testing) {
+    float oxo = 0.0f, oyo = 0.0f, ozo = 0.0f, dxo = 0.0f, dyo = 0.0f, dzo = 0.0f;
+    float rdx = 0.0f, rdy = 0.0f, rdz = 0.0f;
+    int cur = -1;  // a tree node whose children are tested next
+    int sp = 0;    // entries on the lane's stack
+    int pi = 0, xform = -1;
+    int leaf = -1;  // first triangle of the leaf this lane has reached, its tests pending
+    for (;;) {
+      // leaves' triangles in batches, as in walk_meshes
+      const uint64_t waiting = __ballot(leaf >= 0);
+      if (waiting != 0ull && __popcll(waiting) * kLeafWaitDen >= __popcll(__ballot(true)) * kLeafWaitNum) {
+        if (leaf >= 0) {
+          for (int k = leaf;; ++k) {
+            const DTri tr = at(S.tris, k);
+            unit(tri_tests);
+            const float cur_t = isect_triangle(oxo, oyo, ozo, dxo, dyo, dzo, tr, closest);
+            if (cur_t < closest) {
+              closest = cur_t;
+              ci = pi;
+              cst = 4;
+              cso = tr.tri_offset;
+            }
+            if (testing && ci != -1) return true;
+            if (tr.last) break;
+          }
+          leaf = -1;
+        }
+        continue;
+      }
+      if (leaf >= 0) continue;
+      if (cur < 0) {
+        if (sp > 0) {  // the deepest pending second child
+          --sp;
+          const float k = pstack_key[sp * kPairStride];
+          const int r = pstack_ref[sp * kPairStride];
+          if (k < closest) {
+            if (r >= 0) cur = r;
+            else if (r <= -2) leaf = -r - 2;
+          }
+          continue;
+        }
+        // this lane's next mesh: its root box from the DNode
+        if (!mine) break;
+        pi = base + __builtin_ctzll(mine);
+        mine &= mine - 1;
+        unit(prim_tests);
+        const DPrim& pr = S.prims[pi];
+        if (pr.xform != xform) {
+          xform = pr.xform;
+          const float m0 = pr.w2o[0], m1 = pr.w2o[1], m2 = pr.w2o[2], m3 = pr.w2o[3];
+          const float m4 = pr.w2o[4], m5 = pr.w2o[5], m6 = pr.w2o[6], m7 = pr.w2o[7];
+          const float m8 = pr.w2o[8], m9 = pr.w2o[9], m10 = pr.w2o[10], m11 = pr.w2o[11];
+          oxo = m0 * ox + m1 * oy + m2 * oz + m3;
+          oyo = m4 * ox + m5 * oy + m6 * oz + m7;
+          ozo = m8 * ox + m9 * oy + m10 * oz + m11;
+          dxo = m0 * dx + m1 * dy + m2 * dz;
+          dyo = m4 * dx + m5 * dy + m6 * dz;
+          dzo = m8 * dx + m9 * dy + m10 * dz;
+          rdx = 1.0f / dxo;
+          rdy = 1.0f / dyo;
+          rdz = 1.0f / dzo;
+        }
+        const int root = pr.node_root;
+        const DNode nd = at(S.nodes, root);
+        unit(box_tests);
+        const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
+        const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
+        const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
+        const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
+        if (pair_key(tmin, tmax) < closest) {
+          if (nd.tri >= 0) leaf = nd.tri;
+          else if (-nd.tri - 2 != nd.next) cur = root;  // (a tree-coded node whose left is its skip target is empty)
+        }
+        continue;
+      }
+      const DPair P = at(S.pairs, cur);
+      unit(box_tests);
+      unit(box_tests);
+      float key[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const float t0x = (P.mn[0][c] - oxo) * rdx, t0y = (P.mn[1][c] - oyo) * rdy, t0z = (P.mn[2][c] - ozo) * rdz;
+        const float t1x = (P.mx[0][c] - oxo) * rdx, t1y = (P.mx[1][c] - oyo) * rdy, t1z = (P.mx[2][c] - ozo) * rdz;
+        const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
+        const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
+        key[c] = pair_key(tmin, tmax);
+      }
+      int r = -1;
+      if (key[0] < closest) {
+        if (key[1] < closest) {  // the second child waits for the first child's subtree
+          pstack_key[sp * kPairStride] = key[1];
+          pstack_ref[sp * kPairStride] = P.ref[1];
+          ++sp;
+        }
+        r = P.ref[0];
+      } else if (key[1] < closest) {
+        r = P.ref[1];
+      }
+      cur = r >= 0 ? r : -1;
+      leaf = r <= -2 ? -r - 2 : -1;
+    }
+    return false;
+  }
+
   // The primitive loop of doIntersectPrimitives: the closest (or, testing,
   // the first) hit's distance, primitive index, shape type and shape offset.
   //
@@ -320,7 +454,12 @@ struct Tracer {
             const int end = (pr.run_end < base + m ? pr.run_end : base + m) - base;
             const uint64_t run = (end >= 64 ? ~uint64_t{0} : ((uint64_t{1} << end) - 1)) & ~((uint64_t{1} << j) - 1);
             any &= ~run;
-            if (walk_meshes(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing)) return closest;
+            bool done;
+            if constexpr (kPairs)
+              done = walk_meshes_pairs(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing);
+            else
+              done = walk_meshes(cand & run, base, ox, oy, oz, dx, dy, dz, closest, ci, cst, cso, testing);
+            if (done) return closest;
             continue;
           }
           any &= any - 1;

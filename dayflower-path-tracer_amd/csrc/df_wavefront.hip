@@ -269,13 +269,23 @@ __global__ __launch_bounds__(kThreads) void wf_normals(DScene S, DCamera C, DSky
 // Occupancy hint: seven waves per SIMD for the bounces' queries, five for the
 // last launch's (a few registers spilled); measured faster than five, six or
 // eight for the former and six or seven for the latter.
-// kUnits: with the unit counters (Tracer).
-template <int M, bool kEscape, bool kUnits>
+// kUnits: with the unit counters (Tracer).  kPairs: the mesh walks over the
+// two-child records, each lane's stack (S.pair_stack entries of a key and a
+// child) in the launch's dynamic LDS: the keys of the workgroup's lanes, then
+// their children.
+template <int M, bool kEscape, bool kUnits, bool kPairs>
 __global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce, int32_t* noise_count, int32_t* bin_count) {
-  Tracer<M, kUnits> tr(S, C, K, P, kPermTable.v);
+  typedef Tracer<M, kUnits, kPairs> T;
+  T tr(S, C, K, P, kPermTable.v);
+  if constexpr (kPairs) {
+    static_assert(T::kPairStride == kThreads, "one stack slot per lane");
+    extern __shared__ int pair_stack[];
+    tr.pstack_key = (typename T::lds_f*)(reinterpret_cast<float*>(pair_stack) + threadIdx.x);
+    tr.pstack_ref = (typename T::lds_i*)(pair_stack + S.pair_stack * kThreads + threadIdx.x);
+  }
   const bool sky_active = K.sky_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
     bool noisy = false;
@@ -387,11 +397,12 @@ __global__ __launch_bounds__(256) void wf_film(DScene S, DCamera C, DSky K, DPar
 
 // Resident workgroups of a persistent kernel: occupancy x compute units.
 template <class Kernel>
-int resident_blocks(Kernel k) {
+int resident_blocks(Kernel k, size_t dynamic_lds = 0) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 1024;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kThreads, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kThreads, dynamic_lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
   return cus * per_cu;
 }
 
@@ -400,14 +411,28 @@ int resident_blocks(Kernel k) {
 struct Grids {
   int trace, shade, shade_binned, escape, normals;
   int trace_nu, escape_nu;  // the trace kernels without unit counters
+  // the two-child walk's trace kernels (with, without unit counters) for a
+  // per-lane stack of s entries: [s]
+  int trace_pairs[kPairStackMax + 1], trace_pairs_nu[kPairStackMax + 1];
 };
+
+// Bytes of dynamic LDS the two-child walk's trace kernels take for a per-lane
+// stack of s entries.
+inline size_t pair_stack_bytes(int s) { return static_cast<size_t>(s) * 2 * kThreads * sizeof(int32_t); }
 
 template <int M>
 const Grids& grids() {
-  static const Grids g{resident_blocks(wf_trace<M, false, true>), resident_blocks(wf_shade<M, false>),
-                       resident_blocks(wf_shade<M, true>),        resident_blocks(wf_trace<M, true, true>),
-                       resident_blocks(wf_normals<M>),             resident_blocks(wf_trace<M, false, false>),
-                       resident_blocks(wf_trace<M, true, false>)};
+  static const Grids g = [] {
+    Grids r{resident_blocks(wf_trace<M, false, true, false>), resident_blocks(wf_shade<M, false>),
+            resident_blocks(wf_shade<M, true>),               resident_blocks(wf_trace<M, true, true, false>),
+            resident_blocks(wf_normals<M>),                   resident_blocks(wf_trace<M, false, false, false>),
+            resident_blocks(wf_trace<M, true, false, false>), {}, {}};
+    for (int s = 1; s <= kPairStackMax; ++s) {
+      r.trace_pairs[s] = resident_blocks(wf_trace<M, false, true, true>, pair_stack_bytes(s));
+      r.trace_pairs_nu[s] = resident_blocks(wf_trace<M, false, false, true>, pair_stack_bytes(s));
+    }
+    return r;
+  }();
   return g;
 }
 
@@ -419,10 +444,15 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
                                  const DPaths& w, hipStream_t stream, const dfk_marks* marks, int clear, int last) {
   const Grids& g = grids<M>();
   const bool units = p.count_units != 0;
-  const int g_trace = units ? g.trace : g.trace_nu, g_shade = g.shade, g_shade_binned = g.shade_binned,
-            g_escape = units ? g.escape : g.escape_nu, g_normals = g.normals;
-  const auto trace_kernel = units ? wf_trace<M, false, true> : wf_trace<M, false, false>;
-  const auto escape_kernel = units ? wf_trace<M, true, true> : wf_trace<M, true, false>;
+  const bool pairs = p.bvh_pairs != 0 && s.pairs != nullptr && s.pair_stack >= 1 && s.pair_stack <= kPairStackMax;
+  const size_t trace_lds = pairs ? pair_stack_bytes(s.pair_stack) : 0;
+  const int g_trace = pairs ? (units ? g.trace_pairs[s.pair_stack] : g.trace_pairs_nu[s.pair_stack])
+                            : (units ? g.trace : g.trace_nu);
+  const int g_shade = g.shade, g_shade_binned = g.shade_binned, g_escape = units ? g.escape : g.escape_nu,
+            g_normals = g.normals;
+  const auto trace_kernel = pairs ? (units ? wf_trace<M, false, true, true> : wf_trace<M, false, false, true>)
+                                  : (units ? wf_trace<M, false, true, false> : wf_trace<M, false, false, false>);
+  const auto escape_kernel = units ? wf_trace<M, true, true, false> : wf_trace<M, true, false, false>;
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
@@ -453,7 +483,7 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
       // shadow queries of bounce b - 1 (none before the first bounce: that count stays 0)
       int32_t* sc = b > 0 ? shadow_count(b - 1) : shadow_count(bounces);
       mark(kStageTrace, 1);
-      hipLaunchKernelGGL(trace_kernel, dim3(g_trace), dim3(kThreads), 0, stream, s, c, k, p, f, w, sc,
+      hipLaunchKernelGGL(trace_kernel, dim3(g_trace), dim3(kThreads), trace_lds, stream, s, c, k, p, f, w, sc,
                          head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b),
                          bin_count(b));
       mark(kStageTrace, 0);

@@ -93,7 +93,7 @@ class Diff(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int32), ("wave_tile", ctypes.c_int32), ("shade_binned", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("bvh_pairs", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):
@@ -193,6 +193,7 @@ HIP_PROTOTYPES = {
     "df_set_graphs": (ctypes.c_int, [_P, ctypes.c_int32]),
     "df_get_schedule": (ctypes.c_int, [_P, ctypes.POINTER(Schedule)]),
     "df_set_schedule": (ctypes.c_int, [_P, ctypes.POINTER(Schedule)]),
+    "df_get_bvh_pair_stack": (ctypes.c_int, [_P, c_int32_p]),
     "df_batch_last_execute_ms": (ctypes.c_int, [_P, c_float_p]),
     "df_batch_last_error": (ctypes.c_char_p, [_P]),
 }

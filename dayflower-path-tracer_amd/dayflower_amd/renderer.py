@@ -219,19 +219,26 @@ class HIPRendererKernel:
         self._check(self._lib.df_execute(self._ctx, int(passes)), "df_execute")
 
     def schedule(self):
-        """df_get_schedule: {"engine", "wave_tile", "shade_binned"} of the next passes (engine PT_AUTO while the
-        autotuner is still timing its candidates)."""
+        """df_get_schedule: {"engine", "wave_tile", "shade_binned", "bvh_pairs"} of the next passes (engine PT_AUTO
+        while the autotuner is still timing its candidates)."""
         sc = _ffi.Schedule()
         self._check(self._lib.df_get_schedule(self._ctx, ctypes.byref(sc)), "df_get_schedule")
-        return {"engine": sc.engine, "wave_tile": sc.wave_tile, "shade_binned": sc.shade_binned}
+        return {"engine": sc.engine, "wave_tile": sc.wave_tile, "shade_binned": sc.shade_binned,
+                "bvh_pairs": sc.bvh_pairs}
 
-    def setSchedule(self, engine=None, wave_tile=0, shade_binned=0):
+    def setSchedule(self, engine=None, wave_tile=0, shade_binned=0, bvh_pairs=0):
         """df_set_schedule: fix the path tracer's schedule (engine None: back to the parameters' pt_engine)."""
         if engine is None:
             self._check(self._lib.df_set_schedule(self._ctx, None), "df_set_schedule")
         else:
-            sc = _ffi.Schedule(int(engine), int(wave_tile), int(shade_binned), 0)
+            sc = _ffi.Schedule(int(engine), int(wave_tile), int(shade_binned), int(bvh_pairs))
             self._check(self._lib.df_set_schedule(self._ctx, ctypes.byref(sc)), "df_set_schedule")
+
+    def bvhPairStack(self):
+        """df_get_bvh_pair_stack: the per-lane stack depth of the scene's two-child walk (0: not available)."""
+        d = ctypes.c_int32()
+        self._check(self._lib.df_get_bvh_pair_stack(self._ctx, ctypes.byref(d)), "df_get_bvh_pair_stack")
+        return d.value
 
     def setGraphs(self, enabled):
         """df_set_graphs: replay the frame loop's launches from captured HIP graphs (default off)."""

@@ -118,6 +118,12 @@ int df_get_engine(df_ctx* ctx, int32_t* engine);
 int df_get_schedule(df_ctx* ctx, df_schedule* schedule);
 int df_set_schedule(df_ctx* ctx, const df_schedule* schedule);
 
+/* The per-lane stack depth the loaded scene's two-child walk uses (schedule
+ * bvh_pairs), or 0 when the scene is walked over the nodes with either
+ * setting (a tree node with other than two children, or a stack deeper than
+ * 16).  No counterpart in the reference. */
+int df_get_bvh_pair_stack(df_ctx* ctx, int32_t* depth);
+
 /* Frame-loop graphs (no counterpart in the reference): with graphs on
  * (default off: on an MI355X replaying measured 1-7 % slower per pass than
  * launching the same kernels eagerly), df_execute captures the launches of

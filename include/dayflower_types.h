@@ -75,13 +75,17 @@ enum { DF_PT_AUTO = 0, DF_PT_MEGAKERNEL = 1, DF_PT_WAVEFRONT = 2 };
  *   wave_tile    megakernel: the pixels of a wave within its 16x16 tile,
  *                DF_TILE_ROWS (a 16x4 strip) or DF_TILE_SQUARE (8x8);
  *   shade_binned wavefront: 1 = hits are binned by material and each bin
- *                is shaded by code for that material only, 0 = one queue. */
+ *                is shaded by code for that material only, 0 = one queue;
+ *   bvh_pairs    wavefront: 1 = the bounce traces walk the mesh trees over
+ *                64-byte two-child records with a per-lane stack (a scene
+ *                whose trees are not all binary, or need a stack deeper than
+ *                16, is walked over the nodes as with 0). */
 enum { DF_TILE_ROWS = 0, DF_TILE_SQUARE = 1 };
 typedef struct df_schedule {
   int32_t engine;
   int32_t wave_tile;
   int32_t shade_binned;
-  int32_t reserved;
+  int32_t bvh_pairs;
 } df_schedule;
 
 /* Kernel stages, for per-stage counters and timings (df_get_stage_stats).
