@@ -23,6 +23,12 @@ hipError_t dfk_launch_batch(const dfk::DJob* jobs, const dfk::DBatchItem* items,
 
 hipError_t dfk_launch_seed(uint64_t* seeds, uint64_t n, uint64_t master, hipStream_t stream);
 
+// recip_rn against the division for every float with bit pattern in [lo, hi)
+// (both signs): counts into tested / mismatches, smallest mismatch into first
+// (df_check_reciprocal).
+hipError_t dfk_launch_reciprocal_check(uint32_t lo, uint32_t hi, unsigned long long* tested,
+                                       unsigned long long* mismatches, uint32_t* first, hipStream_t stream);
+
 // n rays (6 floats each) through the terrain march of the kernels and
 // through the reference's march over every position (df_check_terrain_march).
 hipError_t dfk_launch_terrain_check(const float* rays, int64_t n, const float* tc_tab, int tn, float* t_skip,

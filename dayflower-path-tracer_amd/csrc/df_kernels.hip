@@ -204,6 +204,11 @@ hipError_t dfk_launch_batch(const dfk::DJob* jobs, const dfk::DBatchItem* items,
 
 namespace dfk {
 
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
 // The reference's terrain march (GPURendererKernel.java:1470-1506) step by
 // step over every position of the table, for checking isect_terrain.
 template <int M>
@@ -238,7 +243,41 @@ __global__ void terrain_check_kernel(const float* rays, int64_t n, const float* 
   t_plain[i] = march_every_position<M>(r[0], r[1], r[2], r[3], r[4], r[5], tc_tab, tn);
 }
 
+// recip_rn against the division for the float bit patterns [lo, hi) of both
+// signs: counts the values tested and the mismatches, and keeps the smallest
+// mismatching pattern.
+__global__ void reciprocal_check_kernel(uint32_t lo, uint32_t hi, unsigned long long* tested,
+                                        unsigned long long* mismatches, uint32_t* first) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  unsigned long long n = 0, bad = 0;
+  for (uint64_t b = lo + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; b < hi; b += stride) {
+    for (uint32_t sign = 0; sign < 2; ++sign) {
+      const uint32_t bits = static_cast<uint32_t>(b) | (sign << 31);
+      const float x = __uint_as_float(bits);
+      const float q = 1.0f / x;
+      ++n;
+      if (__float_as_uint(recip_rn(x)) != __float_as_uint(q)) {
+        ++bad;
+        atomicMin(first, bits);
+      }
+    }
+  }
+  n = wave_sum_u64(n);
+  bad = wave_sum_u64(bad);
+  if (__lane_id() == 0) {
+    atomicAdd(tested, n);
+    if (bad) atomicAdd(mismatches, bad);
+  }
+}
+
 }  // namespace dfk
+
+hipError_t dfk_launch_reciprocal_check(uint32_t lo, uint32_t hi, unsigned long long* tested,
+                                       unsigned long long* mismatches, uint32_t* first, hipStream_t stream) {
+  hipLaunchKernelGGL(dfk::reciprocal_check_kernel, dim3(8192), dim3(256), 0, stream, lo, hi, tested, mismatches,
+                     first);
+  return hipGetLastError();
+}
 
 hipError_t dfk_launch_terrain_check(const float* rays, int64_t n, const float* tc_tab, int tn, float* t_skip,
                                     float* t_plain, uint32_t* evals, int math_mode, hipStream_t stream) {

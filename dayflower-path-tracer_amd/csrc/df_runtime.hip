@@ -1764,6 +1764,30 @@ int df_check_terrain_march(int32_t device, int32_t math_mode, const float* rays,
   return rc;
 }
 
+int df_check_reciprocal(int32_t device, uint64_t* tested, uint64_t* mismatches, uint32_t* first_mismatch) {
+  if (!tested || !mismatches || !first_mismatch) return DF_EINVAL;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return DF_ENODEVICE;
+  if (device < 0 || device >= count) return DF_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return DF_EHIP;
+  // 2^-14 < |x| < 2^126: bit patterns (0x38800000, 0x7E800000) without the sign
+  const uint32_t lo = 0x38800001u, hi = 0x7E800000u;
+  unsigned long long* d = nullptr;
+  if (hipMalloc(&d, 3 * sizeof(unsigned long long)) != hipSuccess) return DF_ENOMEM;
+  const unsigned long long init[3] = {0ull, 0ull, 0xFFFFFFFFull};
+  int rc = DF_OK;
+  unsigned long long out[3] = {0ull, 0ull, 0ull};
+  if (hipMemcpy(d, init, sizeof init, hipMemcpyHostToDevice) != hipSuccess ||
+      dfk_launch_reciprocal_check(lo, hi, d, d + 1, reinterpret_cast<uint32_t*>(d + 2), nullptr) != hipSuccess ||
+      hipMemcpy(out, d, sizeof out, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = DF_EHIP;
+  (void)hipFree(d);
+  *tested = out[0];
+  *mismatches = out[1];
+  *first_mismatch = static_cast<uint32_t>(out[2]);
+  return rc;
+}
+
 int df_set_unit_counters(df_ctx* c, int32_t enabled) {
   if (!c) return DF_EINVAL;
   c->count_units = enabled != 0;

@@ -139,6 +139,8 @@ HIP_PROTOTYPES = {
     "df_device_count": (ctypes.c_int, [c_int32_p]),
     "df_measure_roof": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                        ctypes.POINTER(ctypes.c_double)]),
+    "df_check_reciprocal": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
+                                           ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint32)]),
     "df_check_terrain_march": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int32, c_float_p, ctypes.c_size_t, c_float_p,
                                               c_float_p, ctypes.POINTER(ctypes.c_uint32)]),
     "df_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int32]),

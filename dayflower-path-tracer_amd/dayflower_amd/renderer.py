@@ -69,6 +69,16 @@ def measure_roof(kind, device=0):
     return rate.value, work.value
 
 
+def check_reciprocal(device=0):
+    """df_check_reciprocal: (values tested, mismatches, smallest mismatching bit pattern) of the triangle test's
+    reciprocal against the correctly rounded division over 2^-14 < |x| < 2^126."""
+    tested, bad, first = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+    rc = _ffi.hip_lib().df_check_reciprocal(int(device), ctypes.byref(tested), ctypes.byref(bad), ctypes.byref(first))
+    if rc != _ffi.DF_OK:
+        raise HIPError(rc, "df_check_reciprocal failed")
+    return tested.value, bad.value, first.value
+
+
 def check_terrain_march(rays, math_mode=None, device=0):
     """df_check_terrain_march: (t_skip, t_plain, evals) of the kernels' terrain march and of the reference's march
     over every position for rays of shape (n, 6) (object-space origin, direction)."""

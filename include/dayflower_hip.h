@@ -262,6 +262,15 @@ const char* df_batch_last_error(df_batch* batch);
 int df_check_terrain_march(int32_t device, int32_t math_mode, const float* rays, size_t n, float* t_skip,
                            float* t_plain, uint32_t* evals);
 
+/* ---- Reciprocal check -----------------------------------------------------
+ * (no counterpart in the reference: a self-check for tests.)  The triangle
+ * test takes 1 / det from the hardware reciprocal and one fused correction
+ * when 2^-14 < |det| < 2^126; this compares that with the correctly rounded
+ * division 1.0f / x for every float in that range, both signs, on `device`:
+ * the values tested, the mismatches (must be 0) and the smallest mismatching
+ * bit pattern (0xFFFFFFFF when none). */
+int df_check_reciprocal(int32_t device, uint64_t* tested, uint64_t* mismatches, uint32_t* first_mismatch);
+
 /* ---- Measured roofs -------------------------------------------------------
  * (no counterpart in the reference: measurement only.)  Runs one
  * microbenchmark on `device` and returns its sustained rate: FLOP/s of
