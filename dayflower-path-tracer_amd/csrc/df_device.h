@@ -74,7 +74,7 @@ __device__ __forceinline__ int jf2i(float v) {  // Java (int): truncate, saturat
   if (v <= -2147483648.0f) return (-2147483647 - 1);
   return static_cast<int>(v);
 }
-__device__ __forceinline__ float jrsqrt(float x) { return 1.0f / sqrtf(x); }
+
 
 template <int M>
 struct Math;
@@ -362,6 +362,10 @@ __device__ __forceinline__ float recip_rn(float x) {
   return 1.0f / x;
 }
 
+// Java's 1 / sqrt(x): the correctly rounded square root, then the correctly
+// rounded reciprocal.
+__device__ __forceinline__ float jrsqrt(float x) { return recip_rn(sqrtf(x)); }
+
 __device__ __forceinline__ float isect_triangle(float ox, float oy, float oz, float dx, float dy, float dz,
                                                 const DTri& T, float tmax) {
   const float e0x = T.e0[0], e0y = T.e0[1], e0z = T.e0[2];
@@ -424,7 +428,7 @@ __device__ __forceinline__ float isect_terrain(float ox, float oy, float oz, flo
   const float L1 = fabsf(dx) + fabsf(dy) + fabsf(dz);
   const float L = (fabsf(dy) + sqrtf(dx * dx + dz * dz)) * 1.00000095367431640625f;
   const float margin = 1.0e-5f + 9.5367431640625e-07f * (fabsf(bx) + fabsf(by) + fabsf(bz) + 10.0f * L1);
-  const float inv_l = 1.0f / L;
+  const float inv_l = recip_rn(L);
   // second-order bound: |g''| <= D2 = dx^2 + dz^2 (rounded up, halved)
   const float qb = (dx * dx + dz * dz) * 0.500005f;
   const float gp_err = 1.0e-5f * L1;

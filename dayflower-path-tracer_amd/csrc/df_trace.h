@@ -223,9 +223,9 @@ struct Tracer {
           dxo = m0 * dx + m1 * dy + m2 * dz;
           dyo = m4 * dx + m5 * dy + m6 * dz;
           dzo = m8 * dx + m9 * dy + m10 * dz;
-          rdx = 1.0f / dxo;
-          rdy = 1.0f / dyo;
-          rdz = 1.0f / dzo;
+          rdx = recip_rn(dxo);
+          rdy = recip_rn(dyo);
+          rdz = recip_rn(dzo);
         }
       }
       const DNode nd = at(S.nodes, idx);
@@ -327,9 +327,9 @@ struct Tracer {
           dxo = m0 * dx + m1 * dy + m2 * dz;
           dyo = m4 * dx + m5 * dy + m6 * dz;
           dzo = m8 * dx + m9 * dy + m10 * dz;
-          rdx = 1.0f / dxo;
-          rdy = 1.0f / dyo;
-          rdz = 1.0f / dzo;
+          rdx = recip_rn(dxo);
+          rdy = recip_rn(dyo);
+          rdz = recip_rn(dzo);
         }
         const int root = pr.node_root;
         const DNode nd = at(S.nodes, root);
@@ -563,7 +563,7 @@ struct Tracer {
     const float c1X = isX ? bY - aX : isY ? bZ - aX : bX - aX;
     const float c1Y = isX ? bZ - aY : isY ? bX - aY : bY - aY;
     const float det = b1X * c1Y - b1Y * c1X;
-    const float dr = 1.0f / det;
+    const float dr = recip_rn(det);
     const float bNU = -b1Y * dr, bNV = b1X * dr, bND = (b1Y * a1X - b1X * a1Y) * dr;
     const float cNU = c1Y * dr, cNV = -c1X * dr, cND = (c1X * a1Y - c1Y * a1X) * dr;
     const float hU = isX ? pY : isY ? pZ : pX;
@@ -625,7 +625,7 @@ struct Tracer {
     const float v1X = ox - A[0], v1Y = oy - A[1], v1Z = oz - A[2];
     const float v2X = v1Y * abZ - v1Z * abY, v2Y = v1Z * abX - v1X * abZ, v2Z = v1X * abY - v1Y * abX;
     const float det = abX * v0X + abY * v0Y + abZ * v0Z;
-    const float dr = 1.0f / det;
+    const float dr = recip_rn(det);
     const float bu = (v1X * v0X + v1Y * v0Y + v1Z * v0Z) * dr;
     const float bv = (dx * v2X + dy * v2Y + dz * v2Z) * dr;
     const float bw = 1.0f - bu - bv;
@@ -930,7 +930,7 @@ struct Tracer {
     b += aB * 0.2f * 1.0f;
     const float lX = K.sun_origin[0] - pX, lY = K.sun_origin[1] - pY, lZ = K.sun_origin[2] - pZ;
     const float l2 = lX * lX + lY * lY + lZ * lZ;
-    const float lr = 1.0f / sqrtf(l2);
+    const float lr = recip_rn(sqrtf(l2));
     const float lnX = lX * lr, lnY = lY * lr, lnZ = lZ * lr;
     if (check) intersect_t<false>(pX, pY, pZ, lnX, lnY, lnZ);
     const float t = check ? h.t : DF_INF;
@@ -1254,7 +1254,7 @@ struct Tracer {
         ++rays;  // the any-hit re-test of (o, d): always a hit
         return false;
       }
-      const float pdfr = 1.0f / pdf;
+      const float pdfr = recip_rn(pdf);
       if (vtx) vtx->rr = true;
       aR *= pdfr;
       aG *= pdfr;
@@ -1620,7 +1620,7 @@ struct Tracer {
         const float nX = get_y(pX - DF_EPS, pZ) - get_y(pX + DF_EPS, pZ);
         const float nY = 2.0f * DF_EPS;
         const float nZ = get_y(pX, pZ - DF_EPS) - get_y(pX, pZ + DF_EPS);
-        const float nr = 1.0f / sqrtf(nX * nX + nY * nY + nZ * nZ);
+        const float nr = recip_rn(sqrtf(nX * nX + nY * nY + nZ * nZ));
         const float n = noise.fbm3(8.0f, 0.5f, 0.0f, 1.0f, 16, pX, pY, pZ);
         const float a = saturate(n * 1.0f + 0.1f, 0.0f, 1.0f);
         const int c = phong_shader0(false, pX, pY, pZ, nX * nr, nY * nr, nZ * nr, -dx, -dy, -dz, a, a, a);
