@@ -36,6 +36,11 @@ struct Vtx {
 // 1/6, 1/2, 3/4 and all slower).
 constexpr int kLeafWaitNum = 1, kLeafWaitDen = 4;
 
+// Node visits per step of the mesh walk loop (measured on the 1080p
+// workload: trace 7.79 ms per pass with one, 7.35 with two, 7.18-7.28 with
+// four, 7.27 with six, 7.31 with eight, 7.40 with an inner loop of up to eight).
+constexpr int kVisitsPerStep = 4;
+
 // kUnits: count the unit counters (box, triangle, primitive and sphere
 // tests, terrain height evaluations); the wavefront's trace kernels also come
 // without them, for callers that need only rays and samples (about 3 % of
@@ -228,21 +233,34 @@ struct Tracer {
           rdz = recip_rn(dzo);
         }
       }
-      const DNode nd = at(S.nodes, idx);
-      unit(box_tests);
-      const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
-      const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
-      const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
-      const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
-      // The box is entered when (:1122-1134) its entry or exit lies in
-      // (0.001, closest), unless tmin > tmax (false for NaN); written without
-      // branches, as selects.
-      const int in_min = static_cast<int>(tmin > 0.001f) & static_cast<int>(tmin < closest);
-      const int in_max = static_cast<int>(tmax > 0.001f) & static_cast<int>(tmax < closest);
-      const bool hit = static_cast<bool>(static_cast<int>(!(tmin > tmax)) & (in_min | in_max));
-      const bool tree = nd.tri < 0;
-      leaf = hit && !tree ? nd.tri : -1;                   // a leaf's triangles come before the next visit
-      idx = hit && tree ? -nd.tri - 2 : nd.next;           // tree node: descend to the left child
+      // One node visit: the box test and the next node (or the leaf whose
+      // triangles come first).
+      auto visit = [&]() {
+        const DNode nd = at(S.nodes, idx);
+        unit(box_tests);
+        const float t0x = (nd.mn[0] - oxo) * rdx, t0y = (nd.mn[1] - oyo) * rdy, t0z = (nd.mn[2] - ozo) * rdz;
+        const float t1x = (nd.mx[0] - oxo) * rdx, t1y = (nd.mx[1] - oyo) * rdy, t1z = (nd.mx[2] - ozo) * rdz;
+        const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
+        const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
+        // The box is entered when (:1122-1134) its entry or exit lies in
+        // (0.001, closest), unless tmin > tmax (false for NaN); written without
+        // branches, as selects.
+        const int in_min = static_cast<int>(tmin > 0.001f) & static_cast<int>(tmin < closest);
+        const int in_max = static_cast<int>(tmax > 0.001f) & static_cast<int>(tmax < closest);
+        const bool hit = static_cast<bool>(static_cast<int>(!(tmin > tmax)) & (in_min | in_max));
+        const bool tree = nd.tri < 0;
+        leaf = hit && !tree ? nd.tri : -1;                   // a leaf's triangles come before the next visit
+        idx = hit && tree ? -nd.tri - 2 : nd.next;           // tree node: descend to the left child
+      };
+      // Up to kVisitsPerStep visits per step of the loop while the lane meets
+      // neither a leaf nor its mesh's end: the loop's own work (the leaf
+      // batch ballots, the mesh switch) is paid once per few visits.  Visits
+      // only happen between leaf tests, which alone change `closest`, so each
+      // lane's sequence of tests is unchanged.
+      visit();
+#pragma unroll
+      for (int v = 1; v < kVisitsPerStep; ++v)
+        if (leaf < 0 && idx != -1) visit();
     }
     return false;
   }
