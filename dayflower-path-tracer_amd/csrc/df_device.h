@@ -346,8 +346,6 @@ __device__ __forceinline__ float isect_sphere(float ox, float oy, float oz, floa
   return DF_INF;
 }
 
-// Moller-Trumbore with the reference's epsilons (:1541-1594); e0 = b - a and
-// e1 = c - a come precomputed (the same float subtraction).
 // 1 / x rounded to nearest, as the division 1.0f / x gives it: the hardware
 // reciprocal (within one ulp) and one fused correction r + r (1 - x r),
 // which is the correctly rounded quotient for every float with 2^-14 < |x| <
@@ -366,6 +364,8 @@ __device__ __forceinline__ float recip_rn(float x) {
 // rounded reciprocal.
 __device__ __forceinline__ float jrsqrt(float x) { return recip_rn(sqrtf(x)); }
 
+// Moller-Trumbore with the reference's epsilons (:1541-1594); e0 = b - a and
+// e1 = c - a come precomputed (the same float subtraction).
 __device__ __forceinline__ float isect_triangle(float ox, float oy, float oz, float dx, float dy, float dz,
                                                 const DTri& T, float tmax) {
   const float e0x = T.e0[0], e0y = T.e0[1], e0z = T.e0[2];

@@ -362,31 +362,39 @@ struct Tracer {
         }
         continue;
       }
-      const DPair P = at(S.pairs, cur);
-      unit(box_tests);
-      unit(box_tests);
-      float key[2];
+      // Both children of `cur`; as in walk_meshes, up to kVisitsPerStep of
+      // these per step of the loop while the lane descends into tree nodes.
+      auto expand = [&]() {
+        const DPair P = at(S.pairs, cur);
+        unit(box_tests);
+        unit(box_tests);
+        float key[2];
 #pragma unroll
-      for (int c = 0; c < 2; ++c) {
-        const float t0x = (P.mn[0][c] - oxo) * rdx, t0y = (P.mn[1][c] - oyo) * rdy, t0z = (P.mn[2][c] - ozo) * rdz;
-        const float t1x = (P.mx[0][c] - oxo) * rdx, t1y = (P.mx[1][c] - oyo) * rdy, t1z = (P.mx[2][c] - ozo) * rdz;
-        const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
-        const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
-        key[c] = pair_key(tmin, tmax);
-      }
-      int r = -1;
-      if (key[0] < closest) {
-        if (key[1] < closest) {  // the second child waits for the first child's subtree
-          pstack_key[sp * kPairStride] = key[1];
-          pstack_ref[sp * kPairStride] = P.ref[1];
-          ++sp;
+        for (int c = 0; c < 2; ++c) {
+          const float t0x = (P.mn[0][c] - oxo) * rdx, t0y = (P.mn[1][c] - oyo) * rdy, t0z = (P.mn[2][c] - ozo) * rdz;
+          const float t1x = (P.mx[0][c] - oxo) * rdx, t1y = (P.mx[1][c] - oyo) * rdy, t1z = (P.mx[2][c] - ozo) * rdz;
+          const float tmax = jmin(jmax(t0x, t1x), jmin(jmax(t0y, t1y), jmax(t0z, t1z)));
+          const float tmin = jmax(jmin(t0x, t1x), jmax(jmin(t0y, t1y), jmin(t0z, t1z)));
+          key[c] = pair_key(tmin, tmax);
         }
-        r = P.ref[0];
-      } else if (key[1] < closest) {
-        r = P.ref[1];
-      }
-      cur = r >= 0 ? r : -1;
-      leaf = r <= -2 ? -r - 2 : -1;
+        int r = -1;
+        if (key[0] < closest) {
+          if (key[1] < closest) {  // the second child waits for the first child's subtree
+            pstack_key[sp * kPairStride] = key[1];
+            pstack_ref[sp * kPairStride] = P.ref[1];
+            ++sp;
+          }
+          r = P.ref[0];
+        } else if (key[1] < closest) {
+          r = P.ref[1];
+        }
+        cur = r >= 0 ? r : -1;
+        leaf = r <= -2 ? -r - 2 : -1;
+      };
+      expand();
+#pragma unroll
+      for (int v = 1; v < kVisitsPerStep; ++v)
+        if (cur >= 0) expand();
     }
     return false;
   }
