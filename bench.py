@@ -64,13 +64,13 @@ FLOP_PER = {"box_tests": 24, "tri_tests": 45, "prim_tests": 36, "sphere_tests": 
             "sky_evals": 60}
 BYTES_PER = {"box_tests": 32, "tri_tests": 36, "prim_tests": 48, "sphere_tests": 16}
 PEAK_FP32_TFLOPS = 157.3
-# the timed passes run the trace kernels without the unit counters (kUnits = false)
+# the timed passes run the trace kernels without the unit counters (kUnits = false), node walk (kPairs = false)
 KERNEL_SYMBOLS = {"megakernel": "dfk::render_kernel<0, 2>", "raygen": "dfk::wf_raygen<0>",
-                  "trace": "dfk::wf_trace<0, false, false>", "shade": "dfk::wf_shade<0, false>",
-                  "escape": "dfk::wf_trace<0, true, false>", "film": "dfk::wf_film<0>"}
+                  "trace": "dfk::wf_trace<0, false, false, false>", "shade": "dfk::wf_shade<0, false>",
+                  "escape": "dfk::wf_trace<0, true, false, false>", "film": "dfk::wf_film<0>"}
 KERNEL_NAMES = {"megakernel": "render_kernel (one work-item per pixel)", "raygen": "wf_raygen",
-                "trace": "wf_trace<0,false,false> (a bounce's closest-hit + sun any-hit queries)", "shade": "wf_shade",
-                "escape": "wf_trace<0,true,false> (escape tests + the last sun queries)", "film": "wf_film"}
+                "trace": "wf_trace<0,false,false,false> (a bounce's closest-hit + sun any-hit queries)", "shade": "wf_shade",
+                "escape": "wf_trace<0,true,false,false> (escape tests + the last sun queries)", "film": "wf_film"}
 PEAK_HBM_GBS = 8000.0
 
 
