@@ -64,13 +64,17 @@ FLOP_PER = {"box_tests": 24, "tri_tests": 45, "prim_tests": 36, "sphere_tests": 
             "sky_evals": 60}
 BYTES_PER = {"box_tests": 32, "tri_tests": 36, "prim_tests": 48, "sphere_tests": 16}
 PEAK_FP32_TFLOPS = 157.3
-# the timed passes run the trace kernels without the unit counters (kUnits = false), node walk (kPairs = false)
-KERNEL_SYMBOLS = {"megakernel": "dfk::render_kernel<0, 2>", "raygen": "dfk::wf_raygen<0>",
-                  "trace": "dfk::wf_trace<0, false, false, false>", "shade": "dfk::wf_shade<0, false>",
-                  "escape": "dfk::wf_trace<0, true, false, false>", "film": "dfk::wf_film<0>"}
+# the timed passes run the trace kernels without the unit counters (kUnits = false), node walk (kPairs = false);
+# a stage's launches may run several instantiations of one kernel: the bounce traces run the seven-wave kernel
+# before the schedule's tail bounce and the four-wave kernel from it on
+KERNEL_SYMBOLS = {"megakernel": ["dfk::render_kernel<0, 2>"], "raygen": ["dfk::wf_raygen<0>"],
+                  "trace": ["dfk::wf_trace<0, false, false, false, 7>", "dfk::wf_trace<0, false, false, false, 4>"],
+                  "shade": ["dfk::wf_shade<0, false>"], "escape": ["dfk::wf_trace<0, true, false, false, 4>"],
+                  "film": ["dfk::wf_film<0>"]}
 KERNEL_NAMES = {"megakernel": "render_kernel (one work-item per pixel)", "raygen": "wf_raygen",
-                "trace": "wf_trace<0,false,false,false> (a bounce's closest-hit + sun any-hit queries)", "shade": "wf_shade",
-                "escape": "wf_trace<0,true,false,false> (escape tests + the last sun queries)", "film": "wf_film"}
+                "trace": "wf_trace<0,false,false,false,7|4> (a bounce's closest-hit + sun any-hit queries; seven "
+                         "waves per SIMD before the tail bounce, four from it on)", "shade": "wf_shade",
+                "escape": "wf_trace<0,true,false,false,4> (escape tests + the last sun queries)", "film": "wf_film"}
 PEAK_HBM_GBS = 8000.0
 
 
@@ -88,12 +92,27 @@ def measured_roofs():
     return out
 
 
-def issue_from_profiles(workload, kernel_symbol):
-    """The kernel's VALU issue counters per dispatch from profiles/issue_workload<N>.json, or None."""
+def _per_dispatch(entries, keys):
+    """Means per dispatch over several kernels' entries ({counter: mean per dispatch, "dispatches": n}),
+    weighted by their dispatches, or None when one of them is missing."""
+    if not entries or any(e is None for e in entries):
+        return None
+    n = sum(e["dispatches"] for e in entries)
+    return {k: sum(e[k] * e["dispatches"] for e in entries) / n for k in keys if all(k in e for e in entries)}
+
+
+def issue_from_profiles(workload, kernel_symbols):
+    """VALU issue counters per dispatch of the kernels (one stage's launches) from
+    profiles/issue_workload<N>.json, or None."""
     path = os.path.join(REPO, "profiles", "issue_workload%d.json" % workload)
     if not os.path.exists(path):
         return None
-    return json.load(open(path)).get("kernels", {}).get(kernel_symbol)
+    kernels = json.load(open(path)).get("kernels", {})
+    r = _per_dispatch([kernels.get(k) for k in kernel_symbols],
+                      ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU"))
+    if r and r.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in r:
+        r["active_lane_fraction"] = round(r["SQ_THREAD_CYCLES_VALU"] / (64.0 * r["SQ_ACTIVE_INST_VALU"]), 4)
+    return r
 
 
 def issue_summary(out_path, csv_paths, workload):
@@ -199,14 +218,15 @@ def traffic_summary(out_path, fetch_csv, write_csv, cal_fetch_csv, cal_write_csv
     print(json.dumps(summary))
 
 
-def traffic_from_profiles(workload, kernel_symbol):
-    """HBM bytes per dispatch of the kernel from profiles/traffic_workload<N>.json, or None."""
+def traffic_from_profiles(workload, kernel_symbols):
+    """HBM bytes per dispatch of the kernels (one stage's launches) from profiles/traffic_workload<N>.json, or
+    None."""
     path = os.path.join(REPO, "profiles", "traffic_workload%d.json" % workload)
     if not os.path.exists(path):
         return None
-    data = json.load(open(path))
-    k = data.get("kernels", {}).get(kernel_symbol)
-    return k["hbm_bytes_per_dispatch"] if k else None
+    kernels = json.load(open(path)).get("kernels", {})
+    r = _per_dispatch([kernels.get(k) for k in kernel_symbols], ("hbm_bytes_per_dispatch",))
+    return round(r["hbm_bytes_per_dispatch"]) if r else None
 
 
 def adjoint_measure():
@@ -463,7 +483,7 @@ def main():
         note = ("%.3g FLOP per algorithmic byte, above the FP32 / HBM balance of %.3g: the FP32 peak bounds it "
                 "(no matrix work)" % (intensity, balance))
     roofs = measured_roofs()
-    issue = issue_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, ""))
+    issue = issue_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, []))
     measured = {
         "roofs": roofs,
         "alg_bytes_over_l2": round(achieved_gbs / roofs["l2_gbs"], 4),
@@ -482,7 +502,7 @@ def main():
         "peak": peak,
         "unit": unit,
         "frac": round(achieved / peak, 6),
-        "traffic": traffic_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, "")),
+        "traffic": traffic_from_profiles(args.workload, KERNEL_SYMBOLS.get(dom, [])),
         "kernel": KERNEL_NAMES[dom],
         "note": note,
         "avg_launch_ms": round(avg_launch_s * 1e3, 4),

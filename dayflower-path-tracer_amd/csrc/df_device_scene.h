@@ -175,6 +175,9 @@ struct DParams {
   // wavefront schedule: 1 = the bounce traces walk the meshes over the
   // two-child records (DPair) with a per-lane stack
   int32_t bvh_pairs;
+  // wavefront schedule: the first bounce whose trace launch runs the kernel
+  // for short queues (four waves per SIMD, no spilled registers)
+  int32_t tail_bounce;
   // sRGB constants (image4j RGBColorSpace.SRGB)
   float srgb_break, srgb_gamma_inv, srgb_slope, srgb_slope_match, srgb_segment;
   // differentiable mode (diff_on = 1): continuous albedo / emission

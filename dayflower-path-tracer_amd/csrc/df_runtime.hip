@@ -48,10 +48,10 @@ struct DevBuf {
 // wavefront with one shade queue or with material bins.
 constexpr int kTuneCandidates = 4;
 constexpr df_schedule kCandidates[kTuneCandidates] = {
-    {DF_PT_MEGAKERNEL, DF_TILE_ROWS, 0, 0},
-    {DF_PT_MEGAKERNEL, DF_TILE_SQUARE, 0, 0},
-    {DF_PT_WAVEFRONT, DF_TILE_ROWS, 0, 0},
-    {DF_PT_WAVEFRONT, DF_TILE_ROWS, 1, 0},
+    {DF_PT_MEGAKERNEL, DF_TILE_ROWS, 0, 0, DF_TAIL_BOUNCE_DEFAULT},
+    {DF_PT_MEGAKERNEL, DF_TILE_SQUARE, 0, 0, DF_TAIL_BOUNCE_DEFAULT},
+    {DF_PT_WAVEFRONT, DF_TILE_ROWS, 0, 0, DF_TAIL_BOUNCE_DEFAULT},
+    {DF_PT_WAVEFRONT, DF_TILE_ROWS, 1, 0, DF_TAIL_BOUNCE_DEFAULT},
 };
 
 // One launch sequence captured from a stream and instantiated, with the bytes
@@ -658,6 +658,7 @@ DParams make_params(const df_params& p, int w, int h, int passes, bool clear) {
   d.passes = passes;
   d.clear_first = clear ? 1 : 0;
   d.count_units = 1;
+  d.tail_bounce = DF_TAIL_BOUNCE_DEFAULT;
   // image4j RGBColorSpace.SRGB (breakPoint 0.00304, gamma 2.4) and its
   // derived slope / slopeMatch / segmentOffset (RGBColorSpace.java:148-158).
   const float bp = 0.00304f, gamma = 2.4f;
@@ -894,7 +895,7 @@ void reset_tuning(df_ctx* c) {
 // renderers run as the megakernel).
 df_schedule current_schedule(const df_ctx* c) {
   if (c->sched_fixed || (c->tuned && c->params.pt_engine == DF_PT_AUTO)) return c->sched;
-  df_schedule d{DF_PT_MEGAKERNEL, DF_TILE_ROWS, 0, 0};
+  df_schedule d{DF_PT_MEGAKERNEL, DF_TILE_ROWS, 0, 0, DF_TAIL_BOUNCE_DEFAULT};
   if (c->params.renderer_type == DF_RENDERER_PATH_TRACER && c->params.pt_engine == DF_PT_WAVEFRONT)
     d.engine = DF_PT_WAVEFRONT;
   return d;
@@ -1176,6 +1177,7 @@ int batch_launch(df_batch* b, const BatchPlan& plan, const DJob* djobs, const DB
     DParams par = plan.jobs[j].P;
     par.shade_binned = current_schedule(c).shade_binned;
     par.bvh_pairs = current_schedule(c).bvh_pairs;
+    par.tail_bounce = current_schedule(c).tail_bounce;
     par.count_units = c->count_units ? 1 : 0;
     DF_BHIP(b, dfk_launch_pt_wavefront(c->ds, plan.jobs[j].C, plan.jobs[j].K, par, plan.jobs[j].F, c->paths,
                                        c->params.math_mode, b->stream, nullptr));
@@ -1509,6 +1511,7 @@ int df_execute(df_ctx* c, int32_t passes) {
     par.wave_tile = sc.wave_tile;
     par.count_units = c->count_units ? 1 : 0;
     par.bvh_pairs = wave && sc.bvh_pairs ? 1 : 0;
+    par.tail_bounce = sc.tail_bounce;
     const int math = c->params.math_mode;
     if (wave && graph) {
       // one graph per pass; the wavefront kernels read neither passes nor clear_first
@@ -1614,9 +1617,10 @@ int df_set_schedule(df_ctx* c, const df_schedule* s) {
   }
   if ((s->engine != DF_PT_MEGAKERNEL && s->engine != DF_PT_WAVEFRONT) ||
       (s->wave_tile != DF_TILE_ROWS && s->wave_tile != DF_TILE_SQUARE) ||
-      (s->shade_binned != 0 && s->shade_binned != 1) || (s->bvh_pairs != 0 && s->bvh_pairs != 1))
-    return fail(c, DF_EINVAL, "schedule (engine %d, wave_tile %d, shade_binned %d, bvh_pairs %d)", s->engine,
-                s->wave_tile, s->shade_binned, s->bvh_pairs);
+      (s->shade_binned != 0 && s->shade_binned != 1) || (s->bvh_pairs != 0 && s->bvh_pairs != 1) ||
+      s->tail_bounce < 0)
+    return fail(c, DF_EINVAL, "schedule (engine %d, wave_tile %d, shade_binned %d, bvh_pairs %d, tail_bounce %d)",
+                s->engine, s->wave_tile, s->shade_binned, s->bvh_pairs, s->tail_bounce);
   c->sched = *s;
   c->sched_fixed = true;
   return DF_OK;

@@ -266,15 +266,21 @@ __global__ __launch_bounds__(kThreads) void wf_normals(DScene S, DCamera C, DSky
 // of one launch never touch the same path data: a shadow query writes the
 // radiance (and the sample of a path that ended), a closest-hit query the hit
 // record, an escape test the sample of a path that took every bounce.
-// Occupancy hint: seven waves per SIMD for the bounces' queries, five for the
-// last launch's (a few registers spilled); measured faster than five, six or
-// eight for the former and six or seven for the latter.
+// kWaves: the occupancy hint, waves per SIMD.  Seven (72 registers, some
+// spilled) for launches whose queues fill the GPU: they are bound by VALU
+// issue and the waves hide each other's latency (measured faster than five,
+// six or eight).  Four (128 registers, none spilled) for the launches of the
+// last bounces and the escape tests, whose queues leave most wave slots empty:
+// the launch lasts as long as its slowest waves, and a wave without spill
+// reloads in its walks finishes sooner (1080p pass: trace 7.27 -> 7.10 ms
+// with four from bounce 4 or 5 on, 7.16 from bounce 3 or 6, three waves no
+// faster; escape 0.281 -> 0.265 ms against five).
 // kUnits: with the unit counters (Tracer).  kPairs: the mesh walks over the
 // two-child records, each lane's stack (S.pair_stack entries of a key and a
 // child) in the launch's dynamic LDS: the keys of the workgroup's lanes, then
 // their children.
-template <int M, bool kEscape, bool kUnits, bool kPairs>
-__global__ __launch_bounds__(kThreads, kEscape ? 5 : 7) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+template <int M, bool kEscape, bool kUnits, bool kPairs, int kWaves>
+__global__ __launch_bounds__(kThreads, kWaves) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int first_bounce, int32_t* noise_count, int32_t* bin_count) {
@@ -410,7 +416,8 @@ int resident_blocks(Kernel k, size_t dynamic_lds = 0) {
 // any graph capture: the occupancy queries are not stream work).
 struct Grids {
   int trace, shade, shade_binned, escape, normals;
-  int trace_nu, escape_nu;  // the trace kernels without unit counters
+  int trace_nu, escape_nu;      // the trace kernels without unit counters
+  int trace_tail, trace_tail_nu;  // the last bounces' trace kernels (four waves per SIMD)
   // the two-child walk's trace kernels (with, without unit counters) for a
   // per-lane stack of s entries: [s]
   int trace_pairs[kPairStackMax + 1], trace_pairs_nu[kPairStackMax + 1];
@@ -423,13 +430,14 @@ inline size_t pair_stack_bytes(int s) { return static_cast<size_t>(s) * 2 * kThr
 template <int M>
 const Grids& grids() {
   static const Grids g = [] {
-    Grids r{resident_blocks(wf_trace<M, false, true, false>), resident_blocks(wf_shade<M, false>),
-            resident_blocks(wf_shade<M, true>),               resident_blocks(wf_trace<M, true, true, false>),
-            resident_blocks(wf_normals<M>),                   resident_blocks(wf_trace<M, false, false, false>),
-            resident_blocks(wf_trace<M, true, false, false>), {}, {}};
+    Grids r{resident_blocks(wf_trace<M, false, true, false, 7>),  resident_blocks(wf_shade<M, false>),
+            resident_blocks(wf_shade<M, true>),                   resident_blocks(wf_trace<M, true, true, false, 4>),
+            resident_blocks(wf_normals<M>),                       resident_blocks(wf_trace<M, false, false, false, 7>),
+            resident_blocks(wf_trace<M, true, false, false, 4>),  resident_blocks(wf_trace<M, false, true, false, 4>),
+            resident_blocks(wf_trace<M, false, false, false, 4>), {}, {}};
     for (int s = 1; s <= kPairStackMax; ++s) {
-      r.trace_pairs[s] = resident_blocks(wf_trace<M, false, true, true>, pair_stack_bytes(s));
-      r.trace_pairs_nu[s] = resident_blocks(wf_trace<M, false, false, true>, pair_stack_bytes(s));
+      r.trace_pairs[s] = resident_blocks(wf_trace<M, false, true, true, 7>, pair_stack_bytes(s));
+      r.trace_pairs_nu[s] = resident_blocks(wf_trace<M, false, false, true, 7>, pair_stack_bytes(s));
     }
     return r;
   }();
@@ -448,11 +456,14 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
   const size_t trace_lds = pairs ? pair_stack_bytes(s.pair_stack) : 0;
   const int g_trace = pairs ? (units ? g.trace_pairs[s.pair_stack] : g.trace_pairs_nu[s.pair_stack])
                             : (units ? g.trace : g.trace_nu);
+  const int g_tail = units ? g.trace_tail : g.trace_tail_nu;
   const int g_shade = g.shade, g_shade_binned = g.shade_binned, g_escape = units ? g.escape : g.escape_nu,
             g_normals = g.normals;
-  const auto trace_kernel = pairs ? (units ? wf_trace<M, false, true, true> : wf_trace<M, false, false, true>)
-                                  : (units ? wf_trace<M, false, true, false> : wf_trace<M, false, false, false>);
-  const auto escape_kernel = units ? wf_trace<M, true, true, false> : wf_trace<M, true, false, false>;
+  const auto trace_kernel = pairs ? (units ? wf_trace<M, false, true, true, 7> : wf_trace<M, false, false, true, 7>)
+                                  : (units ? wf_trace<M, false, true, false, 7> : wf_trace<M, false, false, false, 7>);
+  // the last bounces (from p.tail_bounce on; the two-child walk keeps its kernel)
+  const auto tail_kernel = units ? wf_trace<M, false, true, false, 4> : wf_trace<M, false, false, false, 4>;
+  const auto escape_kernel = units ? wf_trace<M, true, true, false, 4> : wf_trace<M, true, false, false, 4>;
   const int bounces = p.depth_max > 1 ? p.depth_max : 1;
   const int ctl_len = wavefront_ctl_len(p.depth_max);
   if (ctl_len > w.ctl_len || static_cast<int64_t>(p.width) * p.height > w.n) return hipErrorInvalidValue;
@@ -482,10 +493,11 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
     for (int b = 0; b < bounces; ++b) {
       // shadow queries of bounce b - 1 (none before the first bounce: that count stays 0)
       int32_t* sc = b > 0 ? shadow_count(b - 1) : shadow_count(bounces);
+      const bool tail = !pairs && b >= p.tail_bounce;
       mark(kStageTrace, 1);
-      hipLaunchKernelGGL(trace_kernel, dim3(g_trace), dim3(kThreads), trace_lds, stream, s, c, k, p, f, w, sc,
-                         head(3 * b), w.queue[b & 1], live(b), head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b),
-                         bin_count(b));
+      hipLaunchKernelGGL(tail ? tail_kernel : trace_kernel, dim3(tail ? g_tail : g_trace), dim3(kThreads),
+                         tail ? 0 : trace_lds, stream, s, c, k, p, f, w, sc, head(3 * b), w.queue[b & 1], live(b),
+                         head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b), bin_count(b));
       mark(kStageTrace, 0);
       mark(kStageShade, 1);  // the noise normal maps count as shading
       hipLaunchKernelGGL(wf_normals<M>, dim3(g_normals), dim3(kThreads), 0, stream, s, c, k, p, w, noise_count(b),

@@ -18,6 +18,7 @@ from .renderer import (  # noqa: F401
     PT_WAVEFRONT,
     TILE_ROWS,
     TILE_SQUARE,
+    TAIL_BOUNCE_DEFAULT,
     check_reciprocal,
     check_terrain_march,
     device_count,

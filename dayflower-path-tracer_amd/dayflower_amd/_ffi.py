@@ -93,7 +93,7 @@ class Diff(ctypes.Structure):
 
 class Schedule(ctypes.Structure):
     _fields_ = [("engine", ctypes.c_int32), ("wave_tile", ctypes.c_int32), ("shade_binned", ctypes.c_int32),
-                ("bvh_pairs", ctypes.c_int32)]
+                ("bvh_pairs", ctypes.c_int32), ("tail_bounce", ctypes.c_int32)]
 
 
 class Params(ctypes.Structure):

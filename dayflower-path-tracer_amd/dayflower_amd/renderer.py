@@ -40,6 +40,7 @@ PT_MEGAKERNEL = 1
 PT_WAVEFRONT = 2
 TILE_ROWS = 0
 TILE_SQUARE = 1
+TAIL_BOUNCE_DEFAULT = 4  # DF_TAIL_BOUNCE_DEFAULT
 
 
 class HIPError(RuntimeError):
@@ -229,19 +230,20 @@ class HIPRendererKernel:
         self._check(self._lib.df_execute(self._ctx, int(passes)), "df_execute")
 
     def schedule(self):
-        """df_get_schedule: {"engine", "wave_tile", "shade_binned", "bvh_pairs"} of the next passes (engine PT_AUTO
-        while the autotuner is still timing its candidates)."""
+        """df_get_schedule: {"engine", "wave_tile", "shade_binned", "bvh_pairs", "tail_bounce"} of the next passes
+        (engine PT_AUTO while the autotuner is still timing its candidates)."""
         sc = _ffi.Schedule()
         self._check(self._lib.df_get_schedule(self._ctx, ctypes.byref(sc)), "df_get_schedule")
         return {"engine": sc.engine, "wave_tile": sc.wave_tile, "shade_binned": sc.shade_binned,
-                "bvh_pairs": sc.bvh_pairs}
+                "bvh_pairs": sc.bvh_pairs, "tail_bounce": sc.tail_bounce}
 
-    def setSchedule(self, engine=None, wave_tile=0, shade_binned=0, bvh_pairs=0):
-        """df_set_schedule: fix the path tracer's schedule (engine None: back to the parameters' pt_engine)."""
+    def setSchedule(self, engine=None, wave_tile=0, shade_binned=0, bvh_pairs=0, tail_bounce=TAIL_BOUNCE_DEFAULT):
+        """df_set_schedule: fix the path tracer's schedule (engine None: back to the parameters' pt_engine).
+        tail_bounce: the first bounce whose trace launch runs the kernel built for short queues."""
         if engine is None:
             self._check(self._lib.df_set_schedule(self._ctx, None), "df_set_schedule")
         else:
-            sc = _ffi.Schedule(int(engine), int(wave_tile), int(shade_binned), int(bvh_pairs))
+            sc = _ffi.Schedule(int(engine), int(wave_tile), int(shade_binned), int(bvh_pairs), int(tail_bounce))
             self._check(self._lib.df_set_schedule(self._ctx, ctypes.byref(sc)), "df_set_schedule")
 
     def bvhPairStack(self):

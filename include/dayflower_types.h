@@ -79,13 +79,22 @@ enum { DF_PT_AUTO = 0, DF_PT_MEGAKERNEL = 1, DF_PT_WAVEFRONT = 2 };
  *   bvh_pairs    wavefront: 1 = the bounce traces walk the mesh trees over
  *                64-byte two-child records with a per-lane stack (a scene
  *                whose trees are not all binary, or need a stack deeper than
- *                16, is walked over the nodes as with 0). */
+ *                16, is walked over the nodes as with 0);
+ *   tail_bounce  wavefront: the first bounce whose trace launch runs the
+ *                trace kernel built for short queues (four waves per SIMD,
+ *                no spilled registers) instead of the one for full queues
+ *                (seven waves per SIMD); 0 = every bounce, at least the
+ *                maximum depth = none; ignored with bvh_pairs.  The
+ *                autotuner's wavefront candidates and the default wavefront
+ *                schedule use DF_TAIL_BOUNCE_DEFAULT. */
 enum { DF_TILE_ROWS = 0, DF_TILE_SQUARE = 1 };
+#define DF_TAIL_BOUNCE_DEFAULT 4
 typedef struct df_schedule {
   int32_t engine;
   int32_t wave_tile;
   int32_t shade_binned;
   int32_t bvh_pairs;
+  int32_t tail_bounce;
 } df_schedule;
 
 /* Kernel stages, for per-stage counters and timings (df_get_stage_stats).

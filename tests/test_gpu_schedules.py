@@ -31,7 +31,8 @@ def _render(sc, p, schedule, passes, seed=42, size=None):
         if schedule is not None:
             k.setSchedule(*schedule)
             got = k.schedule()
-            assert (got["engine"], got["wave_tile"], got["shade_binned"]) == schedule
+            assert (got["engine"], got["wave_tile"], got["shade_binned"], got["bvh_pairs"],
+                    got["tail_bounce"])[:len(schedule)] == tuple(schedule)
         k.execute(passes)
         film, n = k.getFilm()
         return {"film": film, "n": n, "bgra": k.getImageBGRA(), "seeds": k.getSeeds(),
@@ -68,6 +69,54 @@ def test_schedules_agree_in_fast_math_at_full_size(scenes, gpu, cid):
         g = _render(sc, p, schedule, 1)
         for key in ("film", "n", "bgra", "seeds", "prims"):
             np.testing.assert_array_equal(g[key], ref[key], err_msg="%s %s" % (schedule, key))
+
+
+@pytest.mark.parametrize("tail", [0, 2, 99])
+def test_tail_kernel_gives_the_oracles_bits(scenes, gpu, tail):
+    """tail_bounce picks which bounces' trace launches run the kernel built for short queues: 0 = all of them,
+    2 = from the third bounce on, 99 = none (the escape launch always does)."""
+    sc = _small(scenes(3), 160, 90)
+    p = sc.copy_params()
+    p.math_mode = dfa.MATH_EXACT
+    k = dfa.HIPRendererKernel(sc)
+    try:
+        k.setParams(p)
+        k.update(sc.width, sc.height)
+        k.seed(5)
+        seeds0 = k.getSeeds()
+        k.setSchedule(dfa.PT_WAVEFRONT, dfa.TILE_ROWS, 0, 0, tail)
+        assert k.schedule()["tail_bounce"] == tail
+        k.execute(2)
+        film, n = k.getFilm()
+        seeds = k.getSeeds()
+    finally:
+        k.dispose()
+    st = oracle_ffi.OracleState(sc.width, sc.height, seeds0)
+    oracle_ffi.render(sc, p, st, 2)
+    np.testing.assert_array_equal(film, st.film)
+    np.testing.assert_array_equal(n, st.film_samples)
+    np.testing.assert_array_equal(seeds, st.seeds)
+
+
+def test_tail_kernel_agrees_in_fast_math_at_full_size(scenes, gpu):
+    sc = scenes(3)
+    p = sc.copy_params()
+    p.math_mode = dfa.MATH_FAST
+    ref = _render(sc, p, (dfa.PT_WAVEFRONT, dfa.TILE_ROWS, 0, 0, 99), 1)
+    for tail in (0, dfa.TAIL_BOUNCE_DEFAULT):
+        g = _render(sc, p, (dfa.PT_WAVEFRONT, dfa.TILE_ROWS, 0, 0, tail), 1)
+        for key in ("film", "n", "bgra", "seeds", "prims"):
+            np.testing.assert_array_equal(g[key], ref[key], err_msg="tail %d %s" % (tail, key))
+
+
+def test_schedule_rejects_a_negative_tail_bounce(scenes, gpu):
+    sc = _small(scenes(1), 32, 32)
+    k = dfa.HIPRendererKernel(sc)
+    try:
+        with pytest.raises(dfa.HIPError):
+            k.setSchedule(dfa.PT_WAVEFRONT, dfa.TILE_ROWS, 0, 0, -1)
+    finally:
+        k.dispose()
 
 
 def test_autotuner_settles_on_a_candidate(scenes, gpu):
