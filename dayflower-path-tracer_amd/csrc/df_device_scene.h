@@ -276,6 +276,7 @@ struct DPaths {
   int32_t* queue[2];  // live paths of even / odd bounces (kQueueStripes stripes of n entries)
   int32_t* shadow_queue;
   int32_t* noise_queue;  // a bounce's paths whose hit surface has noise normal mapping
+  int32_t* sky_queue;    // the pass's paths that escaped, their sky term pending (every bounce)
   int32_t* shade_bins;   // kShadeBins bins of kQueueStripes stripes of n entries (binned shading)
   float4* noise;         // their three normal-map fBm values (x, y, z)
   int32_t* ctl;       // per-pass queue counts and work heads (groups of kCtlGroup ints)
@@ -303,7 +304,7 @@ constexpr int kShadeBins = 7;
 // and work heads, B = max(1, depth), each a group of per-stripe lines.
 __host__ __device__ inline int wavefront_ctl_len(int depth_max) {
   const int b = depth_max > 1 ? depth_max : 1;
-  return (7 * b + 6 + 2 * kShadeBins * b) * kCtlGroup;
+  return (7 * b + 8 + 2 * kShadeBins * b) * kCtlGroup;
 }
 
 }  // namespace dfk

@@ -98,7 +98,7 @@ struct df_ctx {
 
   // wavefront path state (allocated on first use for the frame size); the
   // shade bins only once a binned schedule runs
-  DevBuf path_f4[12], path_q[4], path_ctl, path_bins;
+  DevBuf path_f4[12], path_q[5], path_ctl, path_bins;
   dfk::DPaths paths{};
 
   // frame-loop graphs: one wavefront pass per (clear, last) flag pair, the
@@ -857,6 +857,7 @@ int ensure_paths(df_ctx* c, int depth_max, bool bins) {
     w.queue[1] = static_cast<int32_t*>(c->path_q[1].p);
     w.shadow_queue = static_cast<int32_t*>(c->path_q[2].p);
     w.noise_queue = static_cast<int32_t*>(c->path_q[3].p);
+    w.sky_queue = static_cast<int32_t*>(c->path_q[4].p);
     w.ctl = static_cast<int32_t*>(c->path_ctl.p);
     w.n = static_cast<int64_t>(n);
     w.ctl_len = ctl_alloc;

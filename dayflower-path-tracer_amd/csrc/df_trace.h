@@ -71,6 +71,9 @@ struct Tracer {
   Rng rng;
   Hit h;
   float tmp[3];
+  // pt_bounce<true>: set when the path missed every primitive and its sky
+  // term is left to a pass of its own (wf_sky)
+  bool sky_pending = false;
   unsigned rays;
   unsigned samples;
   unsigned box_tests;
@@ -1165,6 +1168,11 @@ struct Tracer {
   // the hit surface; a wave draining one bin then takes the same branches of
   // the material switch on every lane.  The arithmetic of every path is the
   // same for every bin.
+  // kDeferSky: a missed path leaves its radiance without the sky term and
+  // sets sky_pending; the caller adds the term m * sky(d) / pi to the
+  // radiance later, as the same single float addition, and so carries no
+  // sky code of its own.
+  template <bool kDeferSky = false>
   __device__ __forceinline__ bool pt_bounce(float o[3], float d[3], float m[3], float p[3], int& depth,
                                             bool sky_active, bool sun_active, bool& shadow, float sd[3],
                                             float pv[3], Vtx* vtx = nullptr, int bin = -1) {
@@ -1177,6 +1185,10 @@ struct Tracer {
     const bool missed = bin == 0 ? true : bin > 0 ? false : (h.t == DF_INF || po == -1);
     if (missed) {
       if (vtx) vtx->miss = true;
+      if constexpr (kDeferSky) {
+        sky_pending = true;
+        return false;
+      }
       sky(sky_active, dx, dy, dz);
       pr += mr * tmp[0] * DF_PI_RECIP;
       pg += mg * tmp[1] * DF_PI_RECIP;

@@ -158,8 +158,9 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
   wave_append(W.queue[0], W.n, W.ctl, valid, gid);
 }
 
-// Occupancy hint: four waves per SIMD (128 VGPRs, a few spills) measured
-// faster than the three an unconstrained allocation allows.
+// Occupancy hint: four waves per SIMD (128 VGPRs, none spilled since the
+// sky terms of escaped paths moved to wf_sky) measured faster than the three
+// an unconstrained allocation allows, and than five or six.
 //
 // kBinned: the paths come in the trace kernel's shade bins (counts `count`
 // and heads `head` are then the bins' groups, kCtlGroup ints apart), drained
@@ -172,7 +173,7 @@ template <int M, bool kBinned>
 __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
-                                                     int bounce) {
+                                                     int32_t* count_sky, int bounce) {
   __shared__ int perm[kPermLen];
   stage_perm(perm);
   Tracer<M> tr(S, C, K, P, perm);
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
   const bool sky_active = K.sky_active == 1;
   const bool sun_active = K.sun_active == 1;
   auto shade = [&](int bin, bool valid, int32_t i) {
-    bool more = false, shadow = false;
+    bool more = false, shadow = false, sky = false;
     if (valid) {
       tr.noise_index = i;
       const float4 o4 = W.ray_o[i], d4 = W.ray_d[i], hv = W.hit[i], m4 = W.thru[i], p4 = W.rad[i];
@@ -191,7 +192,9 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
       tr.rng.s = F.seeds[i];
       int depth = bounce;
       float sd[3], pv[3];
-      more = tr.pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv, nullptr, bin);
+      tr.sky_pending = false;
+      more = tr.template pt_bounce<true>(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv, nullptr, bin);
+      sky = tr.sky_pending;
       F.seeds[i] = tr.rng.s;
       W.rad[i] = f4(p, 0.0f);
       if (shadow) {
@@ -203,12 +206,13 @@ __global__ __launch_bounds__(kThreads, 4) void wf_shade(DScene S, DCamera C, DSk
         W.ray_o[i] = f4(o, 0.0f);
         W.ray_d[i] = f4(d, 0.0f);
         W.thru[i] = f4(m, 0.0f);
-      } else if (!shadow) {
+      } else if (!shadow && !sky) {
         W.sample[i] = f4(p, 0.0f);
       }
     }
     wave_append(q_next, W.n, count_next, valid && more, i);
     wave_append(W.shadow_queue, W.n, count_shadow, valid && shadow, more ? i : (i | kFinish));
+    wave_append(W.sky_queue, W.n, count_sky, valid && sky, i);
   };
   if constexpr (kBinned) {
     for (int b = 0; b < kShadeBins; ++b) {
@@ -283,7 +287,8 @@ template <int M, bool kEscape, bool kUnits, bool kPairs, int kWaves>
 __global__ __launch_bounds__(kThreads, kWaves) void wf_trace(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* shadow_count, int32_t* shadow_head,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
-                                                     int first_bounce, int32_t* noise_count, int32_t* bin_count) {
+                                                     int first_bounce, int32_t* noise_count, int32_t* bin_count,
+                                                     int32_t* sky_count) {
   typedef Tracer<M, kUnits, kPairs> T;
   T tr(S, C, K, P, kPermTable.v);
   if constexpr (kPairs) {
@@ -292,26 +297,20 @@ __global__ __launch_bounds__(kThreads, kWaves) void wf_trace(DScene S, DCamera C
     tr.pstack_key = (typename T::lds_f*)(reinterpret_cast<float*>(pair_stack) + threadIdx.x);
     tr.pstack_ref = (typename T::lds_i*)(pair_stack + S.pair_stack * kThreads + threadIdx.x);
   }
-  const bool sky_active = K.sky_active == 1;
   drain(q, W.n, count, head, [&](bool valid, int32_t i) {
-    bool noisy = false;
+    bool noisy = false, escaped = false;
     int bin = 0;
     if (valid) {
       const float4 o4 = W.ray_o[i], d4 = W.ray_d[i];
       if constexpr (kEscape) {
         // The escape test (:3689-3700) runs beside the last bounce's sun queries,
-        // which may still change this path's radiance: the sample holds the sky
-        // term m * sky / pi (w = 2) or nothing (w = 3), and the film pass adds the
-        // final radiance, the same single float addition the reference makes.
-        const bool hit = tr.template intersect_t<true>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z) < DF_INF;
-        if (!hit) {
-          const float4 m4 = W.thru[i];
-          tr.sky(sky_active, d4.x, d4.y, d4.z);
-          W.sample[i] = make_float4(m4.x * tr.tmp[0] * DF_PI_RECIP, m4.y * tr.tmp[1] * DF_PI_RECIP,
-                                    m4.z * tr.tmp[2] * DF_PI_RECIP, 2.0f);
-        } else {
-          W.sample[i] = make_float4(0.0f, 0.0f, 0.0f, 3.0f);
-        }
+        // which may still change this path's radiance: a path whose ray escapes
+        // goes to the sky pass, which stores its sky term m * sky / pi as the
+        // sample (w = 2); a path whose ray hits something keeps its radiance
+        // (w = 3); the film pass adds the final radiance, the same single float
+        // addition the reference makes.
+        escaped = !(tr.template intersect_t<true>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z) < DF_INF);
+        if (!escaped) W.sample[i] = make_float4(0.0f, 0.0f, 0.0f, 3.0f);
       } else {
         int ci, cst, cso;
         const float t = tr.template query<false>(o4.x, o4.y, o4.z, d4.x, d4.y, d4.z, ci, cst, cso);
@@ -324,6 +323,8 @@ __global__ __launch_bounds__(kThreads, kWaves) void wf_trace(DScene S, DCamera C
     if constexpr (!kEscape) {
       wave_append(W.noise_queue, W.n, noise_count, noisy, i);
       if (P.shade_binned) wave_append_keyed(W.shade_bins, W.n, bin_count, valid, bin, i);
+    } else {
+      wave_append(W.sky_queue, W.n, sky_count, escaped, i);
     }
   });
   drain(W.shadow_queue, W.n, shadow_count, shadow_head, [&](bool valid, int32_t e) {
@@ -345,6 +346,29 @@ __global__ __launch_bounds__(kThreads, kWaves) void wf_trace(DScene S, DCamera C
   unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
   take_counters(tr, cnt);
   flush_counters(cnt, F.counters, kEscape ? kStageEscape : kStageTrace);
+}
+
+// The sky term of the pass's escaped paths (:2127-2236, the miss branch of
+// the path loop and the escape test :3689-3700): m * sky(d) / pi of the ray
+// that escaped, stored as the path's sample (w = 2) for the film pass to add
+// to its radiance.  The shade and escape launches queue these paths instead
+// of evaluating the FP64 sky for a few lanes of a wave each (and carrying its
+// registers): here every lane has one.
+template <int M>
+__global__ __launch_bounds__(kThreads) void wf_sky(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+                                                   const int32_t* count, int32_t* head) {
+  Tracer<M> tr(S, C, K, P, kPermTable.v);
+  const bool sky_active = K.sky_active == 1;
+  drain(W.sky_queue, W.n, count, head, [&](bool valid, int32_t i) {
+    if (!valid) return;
+    const float4 d4 = W.ray_d[i], m4 = W.thru[i];
+    tr.sky(sky_active, d4.x, d4.y, d4.z);
+    W.sample[i] = make_float4(m4.x * tr.tmp[0] * DF_PI_RECIP, m4.y * tr.tmp[1] * DF_PI_RECIP,
+                              m4.z * tr.tmp[2] * DF_PI_RECIP, 2.0f);
+  });
+  unsigned long long cnt[kCounterCount] = {0, 0, 0, 0, 0, 0, 0, 0};
+  take_counters(tr, cnt);
+  flush_counters(cnt, F.counters, kStageShade);
 }
 
 // filmAddColor / filmSetColor, the wireframe overlay and, after the last pass,
@@ -418,6 +442,7 @@ struct Grids {
   int trace, shade, shade_binned, escape, normals;
   int trace_nu, escape_nu;      // the trace kernels without unit counters
   int trace_tail, trace_tail_nu;  // the last bounces' trace kernels (four waves per SIMD)
+  int sky;
   // the two-child walk's trace kernels (with, without unit counters) for a
   // per-lane stack of s entries: [s]
   int trace_pairs[kPairStackMax + 1], trace_pairs_nu[kPairStackMax + 1];
@@ -434,7 +459,7 @@ const Grids& grids() {
             resident_blocks(wf_shade<M, true>),                   resident_blocks(wf_trace<M, true, true, false, 4>),
             resident_blocks(wf_normals<M>),                       resident_blocks(wf_trace<M, false, false, false, 7>),
             resident_blocks(wf_trace<M, true, false, false, 4>),  resident_blocks(wf_trace<M, false, true, false, 4>),
-            resident_blocks(wf_trace<M, false, false, false, 4>), {}, {}};
+            resident_blocks(wf_trace<M, false, false, false, 4>), resident_blocks(wf_sky<M>), {}, {}};
     for (int s = 1; s <= kPairStackMax; ++s) {
       r.trace_pairs[s] = resident_blocks(wf_trace<M, false, true, true, 7>, pair_stack_bytes(s));
       r.trace_pairs_nu[s] = resident_blocks(wf_trace<M, false, false, true, 7>, pair_stack_bytes(s));
@@ -480,6 +505,8 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
   // bounce b's shade bins: kShadeBins count groups, then kShadeBins head groups
   auto bin_count = [&](int b) { return group(7 * bounces + 6 + 2 * kShadeBins * b); };
   auto bin_head = [&](int b) { return group(7 * bounces + 6 + 2 * kShadeBins * b + kShadeBins); };
+  int32_t* sky_count = group(7 * bounces + 6 + 2 * kShadeBins * bounces);
+  int32_t* sky_head = group(7 * bounces + 7 + 2 * kShadeBins * bounces);
   const dim3 pix_block(16, 16);
   const dim3 pix_grid((p.width + 15) / 16, (p.height + 15) / 16);
   auto mark = [&](int stage, int begin) {
@@ -497,7 +524,7 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
       mark(kStageTrace, 1);
       hipLaunchKernelGGL(tail ? tail_kernel : trace_kernel, dim3(tail ? g_tail : g_trace), dim3(kThreads),
                          tail ? 0 : trace_lds, stream, s, c, k, p, f, w, sc, head(3 * b), w.queue[b & 1], live(b),
-                         head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b), bin_count(b));
+                         head(3 * b + 1), b == 0 ? 1 : 0, noise_count(b), bin_count(b), nullptr);
       mark(kStageTrace, 0);
       mark(kStageShade, 1);  // the noise normal maps count as shading
       hipLaunchKernelGGL(wf_normals<M>, dim3(g_normals), dim3(kThreads), 0, stream, s, c, k, p, w, noise_count(b),
@@ -505,11 +532,11 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
       if (p.shade_binned)
         hipLaunchKernelGGL((wf_shade<M, true>), dim3(g_shade_binned), dim3(kThreads), 0, stream, s, c, k, p, f, w,
                            w.queue[b & 1], bin_count(b), bin_head(b), w.queue[(b + 1) & 1], live(b + 1),
-                           shadow_count(b), b);
+                           shadow_count(b), sky_count, b);
       else
         hipLaunchKernelGGL((wf_shade<M, false>), dim3(g_shade), dim3(kThreads), 0, stream, s, c, k, p, f, w,
                            w.queue[b & 1], live(b), head(3 * b + 2), w.queue[(b + 1) & 1], live(b + 1),
-                           shadow_count(b), b);
+                           shadow_count(b), sky_count, b);
       mark(kStageShade, 0);
     }
     // the last bounce's sun queries and the escape tests of the paths that
@@ -518,8 +545,11 @@ hipError_t launch_wavefront_pass(const DScene& s, const DCamera& c, const DSky& 
     mark(kStageEscape, 1);
     hipLaunchKernelGGL(escape_kernel, dim3(g_escape), dim3(kThreads), 0, stream, s, c, k, p, f, w,
                        shadow_count(bounces - 1), head(3 * bounces), w.queue[bounces & 1], live(bounces),
-                       head(3 * bounces + 1), 0, nullptr, nullptr);
+                       head(3 * bounces + 1), 0, nullptr, nullptr, sky_count);
     mark(kStageEscape, 0);
+    mark(kStageShade, 1);  // the sky terms count as shading
+    hipLaunchKernelGGL(wf_sky<M>, dim3(g.sky), dim3(kThreads), 0, stream, s, c, k, p, f, w, sky_count, sky_head);
+    mark(kStageShade, 0);
     mark(kStageFilm, 1);
     hipLaunchKernelGGL(wf_film<M>, pix_grid, pix_block, 0, stream, s, c, k, p, f, w, clear, last);
     mark(kStageFilm, 0);
