@@ -68,11 +68,14 @@ __device__ __forceinline__ double jmind(double a, double b) {
   if (a == 0.0 && b == 0.0 && __builtin_signbit(b)) return b;
   return a <= b ? a : b;
 }
-__device__ __forceinline__ int jf2i(float v) {  // Java (int): truncate, saturate, NaN -> 0
-  if (v != v) return 0;
-  if (v >= 2147483648.0f) return 2147483647;
-  if (v <= -2147483648.0f) return (-2147483647 - 1);
-  return static_cast<int>(v);
+// Java (int): truncate, saturate, NaN -> 0.  Branch-free: the value is
+// clamped into [-2^31, 2^31 - 128] (the largest float below 2^31; fmaxf takes
+// -2^31 for NaN), converted, then the two cases the clamp cannot express are
+// selected (2^31 and above, NaN).
+__device__ __forceinline__ int jf2i(float v) {
+  const int i = static_cast<int>(fminf(fmaxf(v, -2147483648.0f), 2147483520.0f));
+  const int s = v >= 2147483648.0f ? 2147483647 : i;
+  return v != v ? 0 : s;
 }
 
 
@@ -152,22 +155,13 @@ struct Noise {
 
   static __device__ __forceinline__ float grad_dot3(int g, float x, float y, float z) {
     // simplexGradient3 rows: (1,1,0) (-1,1,0) (1,-1,0) (-1,-1,0) (1,0,1) (-1,0,1)
-    // (1,0,-1) (-1,0,-1) (0,1,1) (0,-1,1) (0,1,-1) (0,-1,-1); the products and
-    // sums below are the reference's gx*x + gy*y + gz*z with its zeros.
-    float gx, gy, gz;
-    if (g < 4) {
-      gx = (g & 1) ? -1.0f : 1.0f;
-      gy = (g & 2) ? -1.0f : 1.0f;
-      gz = 0.0f;
-    } else if (g < 8) {
-      gx = (g & 1) ? -1.0f : 1.0f;
-      gy = 0.0f;
-      gz = (g & 2) ? -1.0f : 1.0f;
-    } else {
-      gx = 0.0f;
-      gy = (g & 1) ? -1.0f : 1.0f;
-      gz = (g & 2) ? -1.0f : 1.0f;
-    }
+    // (1,0,-1) (-1,0,-1) (0,1,1) (0,-1,1) (0,1,-1) (0,-1,-1), selected without
+    // branches; the products and sums are the reference's gx*x + gy*y + gz*z
+    // with its zeros.
+    const float s1 = (g & 1) ? -1.0f : 1.0f, s2 = (g & 2) ? -1.0f : 1.0f;
+    const float gx = g < 8 ? s1 : 0.0f;
+    const float gy = g < 4 ? s2 : (g < 8 ? 0.0f : s1);
+    const float gz = g < 4 ? 0.0f : s2;
     return gx * x + gy * y + gz * z;
   }
   static __device__ __forceinline__ float grad_dot2(int g, float x, float y) {
@@ -227,24 +221,14 @@ struct Noise {
     const float x0 = x - (static_cast<float>(i) - t);
     const float y0 = y - (static_cast<float>(j) - t);
     const float z0 = z - (static_cast<float>(k) - t);
-    int i1, j1, k1, i2, j2, k2;
-    if (x0 >= y0) {
-      if (y0 >= z0) {
-        i1 = 1; j1 = 0; k1 = 0; i2 = 1; j2 = 1; k2 = 0;
-      } else if (x0 >= z0) {
-        i1 = 1; j1 = 0; k1 = 0; i2 = 1; j2 = 0; k2 = 1;
-      } else {
-        i1 = 0; j1 = 0; k1 = 1; i2 = 1; j2 = 0; k2 = 1;
-      }
-    } else {
-      if (y0 < z0) {
-        i1 = 0; j1 = 0; k1 = 1; i2 = 0; j2 = 1; k2 = 1;
-      } else if (x0 < z0) {
-        i1 = 0; j1 = 1; k1 = 0; i2 = 0; j2 = 1; k2 = 1;
-      } else {
-        i1 = 0; j1 = 1; k1 = 0; i2 = 1; j2 = 1; k2 = 0;
-      }
-    }
+    // The reference's six orderings (x0 >= y0 ? (y0 >= z0 ? XYZ : x0 >= z0 ?
+    // XZY : ZXY) : (y0 < z0 ? ZYX : x0 < z0 ? YZX : YXZ), comparisons false
+    // for NaN), selected without branches.
+    const bool a = x0 >= y0, b = y0 >= z0, c = x0 >= z0, d = y0 < z0, e = x0 < z0;
+    const bool xyz = a && b, xzy = a && !b && c, zxy = a && !b && !c;
+    const bool zyx = !a && d, yzx = !a && !d && e, yxz = !a && !d && !e;
+    const int i1 = xyz || xzy, j1 = yzx || yxz, k1 = zxy || zyx;
+    const int i2 = xyz || xzy || zxy || yxz, j2 = xyz || zyx || yzx || yxz, k2 = xzy || zxy || zyx || yzx;
     const float x1 = x0 - static_cast<float>(i1) + G3;
     const float y1 = y0 - static_cast<float>(j1) + G3;
     const float z1 = z0 - static_cast<float>(k1) + G3;
