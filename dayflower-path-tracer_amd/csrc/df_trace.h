@@ -442,13 +442,17 @@ struct Tracer {
         for (int u = 0; u < 8; ++u) {
           const int j = j0 + u;
           const DCull cu = load_uniform(S.cull, base + j);  // wave-uniform: scalar loads
+          // This test is no arithmetic of the reference's, only a conservative
+          // filter, so it takes fused multiply-adds: each rounds once where the
+          // products and sums it replaces round twice, so its error stays
+          // within what the DCull allowance was sized for.
           const float lx = cu.c[0] - ox, ly = cu.c[1] - oy, lz = cu.c[2] - oz;
-          const float l2 = lx * lx + ly * ly + lz * lz;
-          const float tca = lx * dx + ly * dy + lz * dz;
+          const float l2 = __builtin_fmaf(lx, lx, __builtin_fmaf(ly, ly, lz * lz));
+          const float tca = __builtin_fmaf(lx, dx, __builtin_fmaf(ly, dy, lz * dz));
           // the enlarged radius r + 1e-4 |c - o| + k |c - o|^2, bounded above
           // without a square root as a + b |c - o|^2 (DCull)
-          const float R = cu.r + cu.k * l2;
-          const float d2 = l2 - tca * tca * ia;
+          const float R = __builtin_fmaf(cu.k, l2, cu.r);
+          const float d2 = __builtin_fmaf(-(tca * tca), ia, l2);
           // comparisons are false for NaN: a ray with non-finite terms is never skipped
           const int miss = static_cast<int>(d2 > R * R) | static_cast<int>(tca < -R * dl);
           const int keep = static_cast<int>(j < m) & ((cu.on == 0) | (miss ^ 1));
