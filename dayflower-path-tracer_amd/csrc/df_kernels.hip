@@ -117,10 +117,12 @@ __global__ __launch_bounds__(256, 4) void render_kernel(DScene S, DCamera C, DSk
 // decreasing estimated cost - and renders all of the tile's passes.  A pixel's
 // passes stay in one work-item, in its own RNG stream, so every job's result
 // is the one render_kernel gives for that scene alone.
-// Occupancy hint: six waves per SIMD (spilling some registers) measured
-// faster on the batch mix than the two an unconstrained allocation allows.
+// Occupancy hint: seven waves per SIMD (spilling some registers) measured
+// fastest on the 256-scene batch mix (1.527 s against 1.594 s at six, 1.554 s
+// at eight, 1.79 s at five), far faster than the two an unconstrained
+// allocation allows.
 template <int M, int R>
-__global__ __launch_bounds__(256, 6) void batch_kernel(const DJob* __restrict__ jobs,
+__global__ __launch_bounds__(256, 7) void batch_kernel(const DJob* __restrict__ jobs,
                                                     const DBatchItem* __restrict__ items, int n_items, int* head) {
   __shared__ int perm[kPermLen];
   __shared__ int next;
