@@ -1471,39 +1471,43 @@ struct Tracer {
     float d[3] = {ray[3], ray[4], ray[5]};
     float m[3] = {1.0f, 1.0f, 1.0f};
     float p[3] = {0.0f, 0.0f, 0.0f};
+    // The sky term of a path whose ray escapes (a miss, or the escape test
+    // after the last bounce) is the path's last addition to its radiance, with
+    // the escaping ray's d and m: it is evaluated after the loop, where the
+    // wave's lanes are together again, instead of at whichever bounce each
+    // lane's path escaped (the FP64 sky then ran once per bounce for the few
+    // lanes escaping there).
+    bool sky_due = false;
     for (;;) {
       intersect_t<false>(o[0], o[1], o[2], d[0], d[1], d[2]);
       if (depth == 0) *prim0 = h.prim;
       bool shadow;
       float sd[3], pv[3];
-      const bool more = pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv);
+      sky_pending = false;
+      const bool more = pt_bounce<true>(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv);
       if (shadow && sun_visible(h.p[0], h.p[1], h.p[2], sd)) {
         p[0] = pv[0];
         p[1] = pv[1];
         p[2] = pv[2];
       }
-      if (!more) break;
-      if (depth >= depth_max) {
-        pt_escape(o, d, m, p, sky_active);
+      if (!more) {
+        sky_due = sky_pending;
+        break;
+      }
+      if (depth >= depth_max) {  // the escape test (:3689-3700)
+        sky_due = !(intersect_t<true>(o[0], o[1], o[2], d[0], d[1], d[2]) < DF_INF);
         break;
       }
     }
-    out[0] = p[0];
-    out[1] = p[1];
-    out[2] = p[2];
-  }
-
-  // The end of a path that took the maximum number of bounces: the sky's
-  // contribution if its last ray escapes (:3689-3700).
-  __device__ __forceinline__ void pt_escape(const float o[3], const float d[3], const float m[3], float p[3],
-                                            bool sky_active) {
-    const bool hit = intersect_t<true>(o[0], o[1], o[2], d[0], d[1], d[2]) < DF_INF;
-    if (!hit) {
+    if (sky_due) {
       sky(sky_active, d[0], d[1], d[2]);
       p[0] += m[0] * tmp[0] * DF_PI_RECIP;
       p[1] += m[1] * tmp[1] * DF_PI_RECIP;
       p[2] += m[2] * tmp[2] * DF_PI_RECIP;
     }
+    out[0] = p[0];
+    out[1] = p[1];
+    out[2] = p[2];
   }
 
   // Ambient occlusion (:3029-3149).
