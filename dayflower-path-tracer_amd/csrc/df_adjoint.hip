@@ -125,6 +125,13 @@ __device__ void replay_path(Tracer<M>& tr, double* g, const DAdjoint& A, const f
   float p[3] = {0.0f, 0.0f, 0.0f};
   PathFactors pf;
   pf.n = 0;
+  // The sky term of a path whose ray escapes is its last contribution; it is
+  // taken after the loop, with the escaping ray's throughput and direction,
+  // where the wave's lanes are together again (the FP64 sky and its dual
+  // numbers then run once, not once per bounce for the few lanes escaping
+  // there).
+  bool sky_due = false;
+  float sky_beta[3] = {0.0f, 0.0f, 0.0f}, sky_d[3] = {0.0f, 0.0f, 0.0f};
   for (;;) {
     tr.template intersect_t<false>(o[0], o[1], o[2], d[0], d[1], d[2]);
     const float beta[3] = {m[0], m[1], m[2]};
@@ -138,8 +145,12 @@ __device__ void replay_path(Tracer<M>& tr, double* g, const DAdjoint& A, const f
     float sd[3], pv[3];
     const bool more = tr.pt_bounce(o, d, m, p, depth, sky_active, sun_active, shadow, sd, pv, &v);
     if (v.miss) {
-      sky_term(tr, g, A, pf, w, beta, din, sky_active);
-      return;
+      sky_due = true;
+      for (int c = 0; c < 3; ++c) {
+        sky_beta[c] = beta[c];
+        sky_d[c] = din[c];
+      }
+      break;
     }
     const bool lit = shadow && tr.sun_visible(tr.h.p[0], tr.h.p[1], tr.h.p[2], sd);
     // emission term beta * e: the light's own parameter, and the albedos in beta
@@ -157,7 +168,7 @@ __device__ void replay_path(Tracer<M>& tr, double* g, const DAdjoint& A, const f
       lds_add(g, sky_row(A) + 1, wba * v.dsun_w[0]);
       lds_add(g, sky_row(A) + 2, wba * v.dsun_w[1]);
     }
-    if (!more) return;
+    if (!more) break;
     if (pf.n < kMaxAdjointDepth) {
       const bool ua = v.uses_albedo && v.aslot >= 0;
 #pragma unroll
@@ -172,12 +183,16 @@ __device__ void replay_path(Tracer<M>& tr, double* g, const DAdjoint& A, const f
     }
     if (depth >= depth_max) {  // the escape test after the last bounce (:3689-3700)
       if (!(tr.template intersect_t<true>(o[0], o[1], o[2], d[0], d[1], d[2]) < DF_INF)) {
-        const float b2[3] = {m[0], m[1], m[2]};
-        sky_term(tr, g, A, pf, w, b2, d, sky_active);
+        sky_due = true;
+        for (int c = 0; c < 3; ++c) {
+          sky_beta[c] = m[c];
+          sky_d[c] = d[c];
+        }
       }
-      return;
+      break;
     }
   }
+  if (sky_due) sky_term(tr, g, A, pf, w, sky_beta, sky_d, sky_active);
 }
 
 // Occupancy hint: three waves per SIMD measured faster than the two an
