@@ -165,17 +165,10 @@ struct Noise {
     return gx * x + gy * y + gz * z;
   }
   static __device__ __forceinline__ float grad_dot2(int g, float x, float y) {
-    float gx, gy;
-    if (g < 4) {
-      gx = (g & 1) ? -1.0f : 1.0f;
-      gy = (g & 2) ? -1.0f : 1.0f;
-    } else if (g < 8) {
-      gx = (g & 1) ? -1.0f : 1.0f;
-      gy = 0.0f;
-    } else {
-      gx = 0.0f;
-      gy = (g & 1) ? -1.0f : 1.0f;
-    }
+    // the x and y components of the same twelve rows, selected without branches
+    const float s1 = (g & 1) ? -1.0f : 1.0f, s2 = (g & 2) ? -1.0f : 1.0f;
+    const float gx = g < 8 ? s1 : 0.0f;
+    const float gy = g < 4 ? s2 : (g < 8 ? 0.0f : s1);
     return gx * x + gy * y;
   }
   static __device__ __forceinline__ int fast_floor(float v) {
@@ -202,12 +195,14 @@ struct Noise {
     const int gi0 = p12(ii + perm[jj]);
     const int gi1 = p12(ii + i1 + perm[jj + j1]);
     const int gi2 = p12(ii + 1 + perm[jj + 1]);
+    // corner terms computed on every lane and selected, as in simplex3
     const float t0 = 0.5f - x0 * x0 - y0 * y0;
-    const float n0 = t0 < 0.0f ? 0.0f : (t0 * t0) * (t0 * t0) * grad_dot2(gi0, x0, y0);
+    const float v0 = (t0 * t0) * (t0 * t0) * grad_dot2(gi0, x0, y0);
     const float t1 = 0.5f - x1 * x1 - y1 * y1;
-    const float n1 = t1 < 0.0f ? 0.0f : (t1 * t1) * (t1 * t1) * grad_dot2(gi1, x1, y1);
+    const float v1 = (t1 * t1) * (t1 * t1) * grad_dot2(gi1, x1, y1);
     const float t2 = 0.5f - x2 * x2 - y2 * y2;
-    const float n2 = t2 < 0.0f ? 0.0f : (t2 * t2) * (t2 * t2) * grad_dot2(gi2, x2, y2);
+    const float v2 = (t2 * t2) * (t2 * t2) * grad_dot2(gi2, x2, y2);
+    const float n0 = t0 < 0.0f ? 0.0f : v0, n1 = t1 < 0.0f ? 0.0f : v1, n2 = t2 < 0.0f ? 0.0f : v2;
     return 70.0f * (n0 + n1 + n2);
   }
 
