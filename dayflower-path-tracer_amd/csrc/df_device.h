@@ -238,19 +238,14 @@ struct Noise {
     const int gi1 = p12(ii + i1 + perm[jj + j1 + perm[kk + k1]]);
     const int gi2 = p12(ii + i2 + perm[jj + j2 + perm[kk + k2]]);
     const int gi3 = p12(ii + 1 + perm[jj + 1 + perm[kk + 1]]);
-    // Each corner's term is computed on every lane and then selected: a
-    // corner is rarely outside the kernel for all 64 lanes of a wave, so a
-    // branch around it saves nothing and costs its exec-mask handling.
     const float t0 = 0.6f - x0 * x0 - y0 * y0 - z0 * z0;
-    const float v0 = (t0 * t0) * (t0 * t0) * grad_dot3(gi0, x0, y0, z0);
+    const float n0 = t0 < 0.0f ? 0.0f : (t0 * t0) * (t0 * t0) * grad_dot3(gi0, x0, y0, z0);
     const float t1 = 0.6f - x1 * x1 - y1 * y1 - z1 * z1;
-    const float v1 = (t1 * t1) * (t1 * t1) * grad_dot3(gi1, x1, y1, z1);
+    const float n1 = t1 < 0.0f ? 0.0f : (t1 * t1) * (t1 * t1) * grad_dot3(gi1, x1, y1, z1);
     const float t2 = 0.6f - x2 * x2 - y2 * y2 - z2 * z2;
-    const float v2 = (t2 * t2) * (t2 * t2) * grad_dot3(gi2, x2, y2, z2);
+    const float n2 = t2 < 0.0f ? 0.0f : (t2 * t2) * (t2 * t2) * grad_dot3(gi2, x2, y2, z2);
     const float t3 = 0.6f - x3 * x3 - y3 * y3 - z3 * z3;
-    const float v3 = (t3 * t3) * (t3 * t3) * grad_dot3(gi3, x3, y3, z3);
-    const float n0 = t0 < 0.0f ? 0.0f : v0, n1 = t1 < 0.0f ? 0.0f : v1;
-    const float n2 = t2 < 0.0f ? 0.0f : v2, n3 = t3 < 0.0f ? 0.0f : v3;
+    const float n3 = t3 < 0.0f ? 0.0f : (t3 * t3) * (t3 * t3) * grad_dot3(gi3, x3, y3, z3);
     return 32.0f * (n0 + n1 + n2 + n3);
   }
 

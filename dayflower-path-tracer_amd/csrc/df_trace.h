@@ -38,8 +38,10 @@ constexpr int kLeafWaitNum = 1, kLeafWaitDen = 4;
 
 // Node visits per step of the mesh walk loop (measured on the 1080p
 // workload: trace 7.79 ms per pass with one, 7.35 with two, 7.18-7.28 with
-// four, 7.27 with six, 7.31 with eight, 7.40 with an inner loop of up to eight).
-constexpr int kVisitsPerStep = 4;
+// four, 7.27 with six, 7.31 with eight, 7.40 with an inner loop of up to
+// eight; measured again once the last bounces had their own kernel: three
+// 7.05, four 7.02-7.08, five 6.98-7.04, six 7.05).
+constexpr int kVisitsPerStep = 5;
 
 // kUnits: count the unit counters (box, triangle, primitive and sphere
 // tests, terrain height evaluations); the wavefront's trace kernels also come

@@ -190,6 +190,24 @@ def test_adjoint_matches_central_differences(gpu, diff_scene):
         k.dispose()
 
 
+def test_adjoint_is_repeatable(gpu, diff_scene):
+    """The same gradient twice from the same seeds: the block sums are reordered by the LDS atomics only, so
+    the entries agree to double rounding (a build whose adjoint read something other than its inputs once
+    gave entries 1-3 % off that also changed from run to run)."""
+    w = h = 64
+    s, k = _kernel(diff_scene, w, h)
+    try:
+        maps, na, ne, nc = _maps(s)
+        target = _target(k, maps, _params(na, ne, 500), w, h)
+        th = _params(na, ne, 501)
+        runs = [_loss(k, maps, th, target, 4, grad=True)[1].copy() for _ in range(3)]
+    finally:
+        k.dispose()
+    scale = float(np.max(np.abs(runs[0])))
+    for g in runs[1:]:
+        np.testing.assert_allclose(g, runs[0], rtol=1e-9, atol=1e-12 * scale)
+
+
 def test_adjoint_sun_angles_through_the_sky(gpu, diff_scene):
     """Sun direct light off: the elevation and azimuth act through the sky model only (smooth in the
     parameters)."""
