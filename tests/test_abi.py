@@ -65,3 +65,32 @@ def test_batch_create_without_device_fails_loudly():
     with pytest.raises(dfa.HIPError) as e:
         dfa.HIPBatchRenderer()
     assert e.value.code == -2
+
+
+def test_struct_layouts_match_the_c_header(tmp_path):
+    """The ctypes mirrors of the ABI structs have the sizes (and, for df_schedule, the field offsets) the C
+    compiler gives the header's declarations."""
+    import shutil
+    import subprocess
+
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    pairs = [("df_schedule", _ffi.Schedule), ("df_sky", _ffi.Sky), ("df_params", _ffi.Params),
+             ("df_diff", _ffi.Diff), ("df_counters", _ffi.Counters), ("df_compiled_scene", _ffi.CompiledScene)]
+    sched_fields = [f[0] for f in _ffi.Schedule._fields_]
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "dayflower_types.h"', "int main(void) {"]
+    for c_name, _ in pairs:
+        src.append('  printf("%%zu\\n", sizeof(%s));' % c_name)
+    for f in sched_fields:
+        src.append('  printf("%%zu\\n", offsetof(df_schedule, %s));' % f)
+    src += ["  return 0;", "}"]
+    (tmp_path / "layout.c").write_text("\n".join(src) + "\n")
+    exe = str(tmp_path / "layout")
+    subprocess.run([cc, "-I", inc, "-o", exe, str(tmp_path / "layout.c")], check=True, capture_output=True)
+    out = [int(v) for v in subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split()]
+    for (c_name, mirror), size in zip(pairs, out):
+        assert ctypes.sizeof(mirror) == size, c_name
+    for f, off in zip(sched_fields, out[len(pairs):]):
+        assert getattr(_ffi.Schedule, f).offset == off, f
