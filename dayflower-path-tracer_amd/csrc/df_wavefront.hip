@@ -158,11 +158,11 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
   wave_append(W.queue[0], W.n, W.ctl, valid, gid);
 }
 
-// Occupancy hint: three waves per SIMD (about 150 VGPRs, none spilled):
-// shading waits on its gathers, and the registers let more of them be in
-// flight per wave (shade 2.20 -> 2.14 ms per 1080p pass against four waves
-// per SIMD, 2.42 with five).  With the FP64 sky still inside it, four had
-// measured faster than three.
+// Occupancy hint: two waves per SIMD: shading waits on its gathers, and the
+// registers let more of them be in flight per wave (shade stage 2.20 -> 2.14
+// -> 2.13 ms per 1080p pass from four waves per SIMD to three to two, 2.42
+// with five).  With the FP64 sky still inside it, four had measured faster
+// than three.
 //
 // kBinned: the paths come in the trace kernel's shade bins (counts `count`
 // and heads `head` are then the bins' groups, kCtlGroup ints apart), drained
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void wf_raygen(DScene S, DCamera C, DSky K, DP
 // seven copies made the kernel 93k instructions against 18k, past what the
 // instruction cache holds.)
 template <int M, bool kBinned>
-__global__ __launch_bounds__(kThreads, 3) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
+__global__ __launch_bounds__(kThreads, 2) void wf_shade(DScene S, DCamera C, DSky K, DParams P, DFrame F, DPaths W,
                                                      const int32_t* q, const int32_t* count, int32_t* head,
                                                      int32_t* q_next, int32_t* count_next, int32_t* count_shadow,
                                                      int32_t* count_sky, int bounce) {
